@@ -1,0 +1,229 @@
+/*
+ * tsc_hotpath.h -- C-ABI of the MI355X (gfx950) hot path of the two-site
+ * SU(2) DMRG for the t1-t2-J1-J2 t-J model on triangular cylinders.
+ *
+ * Everything here is plain C: pointers, sizes, int status codes (0 = ok).
+ * Device buffers are owned by a context; host code hands over host arrays
+ * (column-major complex128, Fortran layout) and gets host arrays back, or
+ * keeps vectors resident on the device between calls through the
+ * tsc_vec_* handles.
+ *
+ * Reference interfaces replaced (hyx15903/TSC_Tri_t1t2J1J2, DMRG_code/):
+ *   model_wave        get_sys.f90:6915-6922  -> tsc_plan_* + tsc_hpsi
+ *   hmerge            get_sys.f90:7398-7402  -> tsc_block_axpy (merged op)
+ *   lanczos_memo_new  eigen.f90:4-12         -> tsc_lanczos
+ *   wave_product / wave_norm / Gram_Schmidt_process
+ *                     block.f90:1092-1150,1251-1289 -> tsc_vec_dot_re,
+ *                                            tsc_vec_norm, used by tsc_lanczos
+ *   Get_density_matrix_sys / _env
+ *                     den_mat.f90:4-99       -> tsc_density_matrix
+ *   Get_truncation_operator
+ *                     den_mat.f90:184-418    -> tsc_truncation_operator
+ *   update_trun_dia / update_trun_ndia
+ *                     block.f90:3465-3683    -> tsc_update_trun
+ *   W6JS              block.f90:3744-3795    -> tsc_w6j
+ *
+ * Threading: a context is single-stream and not re-entrant, as the
+ * reference's globals are. Errors: nonzero return, message from
+ * tsc_last_error(); the Fortran shim turns them into `error stop`.
+ */
+#ifndef TSC_HOTPATH_H
+#define TSC_HOTPATH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TSC_OK 0
+#define TSC_EINVAL 1   /* bad argument / inconsistent descriptor        */
+#define TSC_EHIP 2     /* HIP runtime error                              */
+#define TSC_ENOMEM 3   /* device allocation failed                       */
+#define TSC_ESTATE 4   /* call out of order (e.g. no plan)               */
+
+/* ---- precision policies for the grouped contraction -------------------- */
+#define TSC_PREC_F64 0    /* complex FP64, FP64 MFMA                      */
+#define TSC_PREC_F32 1    /* complex FP32 products, FP32 MFMA, FP32 acc   */
+#define TSC_PREC_BF16X3 2 /* BF16x3 split products, FP32 accumulation     */
+
+/* ---- operand modes of a contribution ----------------------------------- */
+#define TSC_OP_N 0      /* A                   */
+#define TSC_OP_T 1      /* A^T                 */
+#define TSC_OP_C 2      /* A^H (conj-transpose)*/
+#define TSC_OP_CONJ 3   /* conj(A)             */
+#define TSC_OP_I 4      /* identity (no load)  */
+
+/* ---- buffers a contribution can address -------------------------------- */
+#define TSC_BUF_OPS 0   /* operator arena (uploaded with the plan)        */
+#define TSC_BUF_IN 1    /* input vector psi                               */
+#define TSC_BUF_MID 2   /* stage-1 intermediates (owned by the plan)      */
+#define TSC_BUF_OUT 3   /* output vector H psi                            */
+
+/*
+ * One term of a grouped contraction:
+ *     C[out block] (+)= coef * op_a(A) * op_b(B)
+ * A is rows(C) x k after op_a, B is k x cols(C) after op_b; both stored
+ * column-major at element offsets a_off / b_off of buffers a_buf / b_buf
+ * with leading dimensions lda / ldb (complex elements). With op_a = I the
+ * term is coef*op_b(B) (k = rows(C)); with op_b = I it is coef*op_a(A).
+ */
+typedef struct tsc_contrib {
+  int64_t a_off, b_off;
+  int32_t lda, ldb;
+  int32_t k;
+  int8_t a_buf, b_buf, a_op, b_op;
+  double coef_re, coef_im;
+} tsc_contrib;
+
+/*
+ * One output block: an m x n window (leading dimension ldc) of buffer
+ * c_buf at element offset c_off, written as the sum of contributions
+ * [cbeg, cend). mode 0 overwrites the window, mode 1 adds to it.
+ */
+typedef struct tsc_outblock {
+  int64_t c_off;
+  int32_t ldc, m, n;
+  int32_t c_buf, cbeg, cend, mode;
+} tsc_outblock;
+
+typedef struct tsc_ctx tsc_ctx;
+typedef struct tsc_plan tsc_plan;
+typedef struct tsc_vec tsc_vec;
+
+const char *tsc_last_error(void);
+int tsc_version(void);
+
+int tsc_ctx_create(int device, tsc_ctx **out);
+int tsc_ctx_destroy(tsc_ctx *ctx);
+int tsc_ctx_sync(tsc_ctx *ctx);
+
+/* ---- resident vectors (psi, Lanczos vectors) ---------------------------- */
+int tsc_vec_create(tsc_ctx *ctx, int64_t n, tsc_vec **out);
+int tsc_vec_destroy(tsc_vec *v);
+int tsc_vec_upload(tsc_vec *v, const double *host_c128);
+int tsc_vec_download(const tsc_vec *v, double *host_c128);
+/* Re<u|v> as in wave_product (block.f90:1092-1108: real part only) */
+int tsc_vec_dot_re(const tsc_vec *u, const tsc_vec *v, double *out);
+int tsc_vec_norm(const tsc_vec *v, double *out);
+/* device pointer of a vector (complex128, n elements) */
+void *tsc_vec_dptr(tsc_vec *v);
+/* v = seeded complex Gaussians (same generator as the operator fill) */
+int tsc_vec_fill_gaussian(tsc_vec *v, uint64_t seed, uint64_t key, double sigma);
+/* y = a*x + b*y (real a, b), elementwise */
+int tsc_vec_axpby(tsc_vec *y, double a, const tsc_vec *x, double b);
+/* y = x (same length) */
+int tsc_vec_copy(tsc_vec *y, const tsc_vec *x);
+
+/*
+ * ---- grouped two-stage contraction plan (the H.psi engine) --------------
+ * Stage 1 writes intermediates into the plan's MID buffer (mid_elems
+ * complex elements), stage 2 writes the output vector. The operator arena
+ * (ops_elems complex128 values) is uploaded once and stays resident.
+ * Replaces the per-call sector scans of model_wave (get_sys.f90:6915-7393)
+ * and the kernels it dispatches (get_sys.f90:1783-5204).
+ */
+int tsc_plan_create(tsc_ctx *ctx, const double *ops_host_c128, int64_t ops_elems,
+                    int64_t mid_elems, const tsc_contrib *c1, int32_t nc1,
+                    const tsc_outblock *o1, int32_t no1, const tsc_contrib *c2,
+                    int32_t nc2, const tsc_outblock *o2, int32_t no2,
+                    int64_t in_elems, int64_t out_elems, tsc_plan **out);
+int tsc_plan_destroy(tsc_plan *p);
+/* replace the operator arena values (same layout) */
+int tsc_plan_update_ops(tsc_plan *p, const double *ops_host_c128);
+/*
+ * Fill arena elements [off, off+count) with seeded complex Gaussians
+ * (E|z|^2 = sigma^2) from a counter-based generator, on the device. Used
+ * to build large synthetic superblocks without a host copy; with
+ * ops_host_c128 = NULL at plan creation the arena starts zeroed.
+ */
+int tsc_plan_fill_ops_gaussian(tsc_plan *p, int64_t off, int64_t count, uint64_t seed,
+                               uint64_t key, double sigma);
+/* algorithmic FLOP count of one application (8 flops per complex MAC) */
+int tsc_plan_flops(const tsc_plan *p, double *flops);
+/* out = H in, on the context stream; precision = TSC_PREC_* */
+int tsc_hpsi(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precision);
+/* same with host arrays (uploads, runs, downloads) */
+int tsc_hpsi_host(tsc_plan *p, const double *in_c128, double *out_c128, int precision);
+/* capture one application in a hipGraph and replay it from then on */
+int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precision);
+/* device time (ms) of the last stage-1 / stage-2 launches, HIP events */
+int tsc_plan_last_times(const tsc_plan *p, float *ms_stage1, float *ms_stage2);
+
+/*
+ * ---- Lanczos (lanczos_memo_new, eigen.f90:4-199) -------------------------
+ * Krylov vectors stay resident; alpha/beta/dots are FP64; tridiagonal
+ * solves run on the host in FP64. psi holds the start vector on entry and
+ * the normalised Ritz vector on exit. precision selects H.psi products
+ * for the Krylov build; refine_iters > 0 appends that many FP64 Lanczos
+ * steps restarted from the mixed-precision Ritz vector.
+ * info[0] = total H.psi calls, info[1] = outer restarts used.
+ */
+int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer,
+                int precision, int refine_iters, double *energy, int32_t *info);
+
+/*
+ * ---- density matrix, eigensolve, truncation (den_mat.f90) ---------------
+ * psi sectors are column-major (sys_dim x env_dim) blocks laid end to end.
+ * side 0: rho_s = psi_s psi_s^H (sys, den_mat.f90:41-43);
+ * side 1: rho_s = psi_s^H psi_s (env, den_mat.f90:89-91).
+ * rho_out holds sum_s d_s^2 complex128 values (d_s = rows of rho_s).
+ */
+int tsc_density_matrix(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec,
+                       const int32_t *sys_dim, const int32_t *env_dim, int side,
+                       double *rho_out_c128);
+
+/*
+ * Get_truncation_operator (den_mat.f90:184-418) on the device:
+ * per-sector Hermitian eigensolve of rho (ascending), global descending
+ * sort (the reference's quicksort order), kept-state count from
+ * kept_min / kept_max / trun_err, per-sector kept counts, and U (the top
+ * kept eigenvectors of each sector, ascending order, column-major
+ * d_s x kept_s, laid end to end over sectors with kept_s > 0).
+ *   twos[s]        2S of sector s (entropy weights 2S+1)
+ *   eigval_out     sum_s d_s eigenvalues, ascending per sector
+ *   kept_out[s]    kept states of sector s
+ *   u_out          sum_s d_s*kept_s complex128 values
+ *   scalars_out    [0]=num_state [1]=discarded weight [2]=entropy (kept
+ *                  spectrum, Entropys) [3]=entropy1 (U(1)-resolved)
+ */
+int tsc_truncation_operator(tsc_ctx *ctx, int32_t nsec, const int32_t *dim,
+                            const int32_t *twos, const double *rho_c128,
+                            int32_t kept_min, int32_t kept_max, double trun_err,
+                            double *eigval_out, int32_t *kept_out, double *u_out_c128,
+                            double *scalars_out);
+
+/*
+ * Get_density_matrix_sys/_env followed by Get_truncation_operator without
+ * a host round trip of rho (den_mat.f90:4-99 then :184-418): rho is built
+ * and diagonalised on the device from the resident psi; only eigenvalues
+ * and the kept eigenvectors come back. Arguments as in the two calls above
+ * (dims per sector from sys_dim or env_dim according to side).
+ */
+int tsc_rho_truncate(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, const int32_t *sys_dim,
+                     const int32_t *env_dim, const int32_t *twos, int side, int32_t kept_min,
+                     int32_t kept_max, double trun_err, double *eigval_out, int32_t *kept_out,
+                     double *u_out_c128, double *scalars_out);
+
+/*
+ * update_trun_dia / update_trun_ndia (block.f90:3465-3683):
+ * eff_i = U_lhs(i)^H . O_i . U_rhs(i) for nblk operator sub-blocks.
+ * For each i: O_i is row_i x col_i at ori_off[i] (complex elements in
+ * ori_c128), U_lhs is row_i x kl_i at ul_off[i], U_rhs is col_i x kr_i at
+ * ur_off[i] (offsets into u_c128), result kl_i x kr_i at eff_off[i].
+ */
+int tsc_update_trun(tsc_ctx *ctx, int32_t nblk, const int32_t *row, const int32_t *col,
+                    const int32_t *kl, const int32_t *kr, const int64_t *ori_off,
+                    const int64_t *ul_off, const int64_t *ur_off, const int64_t *eff_off,
+                    const double *ori_c128, int64_t ori_elems, const double *u_c128,
+                    int64_t u_elems, double *eff_c128, int64_t eff_elems, int precision);
+
+/* Racah 6j symbol with doubled arguments (W6JS, block.f90:3744-3795) */
+double tsc_w6j(int32_t j1, int32_t j2, int32_t j3, int32_t l1, int32_t l2, int32_t l3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSC_HOTPATH_H */

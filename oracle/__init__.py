@@ -1,0 +1,22 @@
+"""CPU oracle for the DMRG hot path -- test infrastructure only.
+
+A restatement, in numpy and plain Python, of the reference's algorithms
+(hyx15903/TSC_Tri_t1t2J1J2, Fortran, DMRG_code/). Only tests/, the smoke
+check of __graft_entry__ and the cpu_baseline leg of bench.py use it, and
+only as the checker. The product (tsc_tri_t1t2j1j2_amd) never imports it.
+
+Pinning (the reference itself cannot be built here: it needs Intel MKL,
+which this image lacks, plus source edits to its compile-time lattice
+parameters):
+  wigner.w6j          exact rational Racah sum; checked against sympy's
+                      wigner_6j, closed forms and Regge symmetries
+  truncation.*        numpy eigh; invariants (Hermitian rho, trace 1,
+                      orthonormal U, discarded weight = 1 - kept weight),
+                      brute-force selection on tiny spectra
+  lanczos.*           exact ground energies of dense Hermitian matrices
+                      (numpy eigh)
+  uou.update_trun     identity U maps O to O; equals U^H O U elementwise
+  synth.hpsi          direct per-term product definition (no reordering)
+  su2 / tj_model      exact diagonalisation of small t-J clusters in the
+                      U(1) Fock basis (see tests/test_oracle_physics.py)
+"""

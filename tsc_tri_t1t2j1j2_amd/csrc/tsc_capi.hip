@@ -1,0 +1,675 @@
+// C-ABI of the hot path: contexts, grouped-contraction plans (H.psi),
+// and the Lanczos driver.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "tsc_internal.h"
+
+namespace tsc {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+__global__ void f64_to_f32_kernel(const double2 *__restrict__ src, float2 *__restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = src[i];
+    dst[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric tridiagonal eigen-solve for the Lanczos matrix (DSTEV's role,
+// eigen.f90:99,117,141): lowest eigenvalue by Sturm-sequence bisection,
+// its eigenvector by inverse iteration with a pivoted tridiagonal solve.
+static int sturm_below(const double *d, const double *e, int n, double x) {
+  int cnt = 0;
+  double q = d[0] - x;
+  const double tiny = 1e-300;
+  if (q < 0) ++cnt;
+  for (int i = 1; i < n; ++i) {
+    if (std::fabs(q) < tiny) q = (q < 0 ? -tiny : tiny);
+    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+double tridiag_lowest(const double *d, const double *e, int n) {
+  if (n == 1) return d[0];
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  const double scale = std::max(std::fabs(lo), std::fabs(hi));
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_below(d, e, n, mid) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+    if (hi - lo <= 2.0 * 2.2e-16 * scale) break;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// eigenvector of the lowest eigenvalue lam (unit norm, first nonzero
+// component made positive)
+void tridiag_vector(const double *d, const double *e, int n, double lam, double *z) {
+  if (n == 1) {
+    z[0] = 1.0;
+    return;
+  }
+  double scale = 0.0;
+  for (int i = 0; i < n; ++i) scale = std::max(scale, std::fabs(d[i]) + (i < n - 1 ? std::fabs(e[i]) : 0.0));
+  const double shift = lam - 4.0 * 2.2e-16 * std::max(scale, 1e-300);
+  std::vector<double> b(n, 1.0), dl(n), dd(n), du(n), du2(n);
+  for (int it = 0; it < 4; ++it) {
+    // (T - shift I) x = b, Gaussian elimination with partial pivoting
+    for (int i = 0; i < n; ++i) {
+      dd[i] = d[i] - shift;
+      if (i < n - 1) {
+        dl[i] = e[i];
+        du[i] = e[i];
+      }
+      du2[i] = 0.0;
+    }
+    std::vector<double> x(b);
+    for (int i = 0; i < n - 1; ++i) {
+      if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+        double f = (dd[i] != 0.0) ? dl[i] / dd[i] : 0.0;
+        dd[i + 1] -= f * du[i];
+        x[i + 1] -= f * x[i];
+        dl[i] = 0.0;
+      } else {
+        double f = dd[i] / dl[i];
+        dd[i] = dl[i];
+        double t = dd[i + 1];
+        dd[i + 1] = du[i] - f * t;
+        if (i < n - 2) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du[i + 1];
+        }
+        du[i] = t;
+        double tx = x[i];
+        x[i] = x[i + 1];
+        x[i + 1] = tx - f * x[i + 1];
+      }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = x[i];
+      if (i < n - 1) s -= du[i] * x[i + 1];
+      if (i < n - 2) s -= du2[i] * x[i + 2];
+      double piv = dd[i];
+      if (std::fabs(piv) < 1e-300) piv = 1e-300;
+      x[i] = s / piv;
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < n; ++i) b[i] = x[i] / nrm;
+  }
+  int first = 0;
+  while (first < n - 1 && std::fabs(b[first]) < 1e-300) ++first;
+  const double sg = b[first] < 0 ? -1.0 : 1.0;
+  for (int i = 0; i < n; ++i) z[i] = sg * b[i];
+}
+
+}  // namespace tsc
+
+using namespace tsc;
+
+
+extern "C" const char *tsc_last_error(void) { return g_err.c_str(); }
+extern "C" int tsc_version(void) { return 1; }
+
+extern "C" int tsc_ctx_create(int device, tsc_ctx **out) {
+  TSC_REQUIRE(out, "tsc_ctx_create: null out");
+  int ndev = 0;
+  TSC_HIP_CHECK(hipGetDeviceCount(&ndev));
+  TSC_REQUIRE(device >= 0 && device < ndev, "tsc_ctx_create: no such device");
+  TSC_HIP_CHECK(hipSetDevice(device));
+  tsc_ctx *c = new tsc_ctx();
+  c->device = device;
+  TSC_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  TSC_HIP_CHECK(hipMalloc((void **)&c->d_red, sizeof(double) * kRedBlocks * kMaxVecs));
+  TSC_HIP_CHECK(hipHostMalloc((void **)&c->h_red, sizeof(double) * kRedBlocks * kMaxVecs, hipHostMallocDefault));
+  TSC_HIP_CHECK(hipMalloc((void **)&c->d_ptrs, sizeof(void *) * kMaxVecs));
+  TSC_HIP_CHECK(hipMalloc((void **)&c->d_ptrs2, sizeof(void *) * kMaxVecs));
+  TSC_HIP_CHECK(hipMalloc((void **)&c->d_coef, sizeof(double) * kMaxVecs));
+  TSC_HIP_CHECK(hipHostMalloc((void **)&c->h_ptrs, sizeof(void *) * kMaxVecs, hipHostMallocDefault));
+  TSC_HIP_CHECK(hipHostMalloc((void **)&c->h_coef, sizeof(double) * kMaxVecs, hipHostMallocDefault));
+  *out = c;
+  return TSC_OK;
+}
+
+extern "C" int tsc_ctx_destroy(tsc_ctx *c) {
+  if (!c) return TSC_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  hipFree(c->d_red);
+  hipHostFree(c->h_red);
+  hipFree(c->d_ptrs);
+  hipFree(c->d_ptrs2);
+  hipFree(c->d_coef);
+  hipHostFree(c->h_ptrs);
+  hipHostFree(c->h_coef);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return TSC_OK;
+}
+
+extern "C" int tsc_ctx_sync(tsc_ctx *c) {
+  TSC_REQUIRE(c, "tsc_ctx_sync: null ctx");
+  TSC_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return TSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plan construction: validate every contribution against its buffer, build
+// the tile lists (largest work first) and upload everything.
+namespace {
+
+struct StageCheck {
+  int64_t size[4];
+};
+
+int check_stage(const tsc_contrib *c, int32_t nc, const tsc_outblock *o, int32_t no, const StageCheck &sz,
+                int out_buf, const int *readable, double *flops) {
+  for (int32_t b = 0; b < no; ++b) {
+    const tsc_outblock &ob = o[b];
+    TSC_REQUIRE(ob.c_buf == out_buf, "output block writes the wrong buffer");
+    TSC_REQUIRE(ob.m >= 0 && ob.n >= 0 && ob.ldc >= std::max(1, ob.m), "output block shape");
+    TSC_REQUIRE(ob.cbeg >= 0 && ob.cend >= ob.cbeg && ob.cend <= nc, "output block contribution range");
+    TSC_REQUIRE(ob.mode == 0 || ob.mode == 1, "output block mode");
+    TSC_REQUIRE((int64_t)ob.m * ob.n <= (int64_t)1 << 40, "output block too large");
+    TSC_REQUIRE((ob.m + kTM - 1) / kTM < 32768 && (ob.n + kTN - 1) / kTN < 32768, "output block too many tiles");
+    if (ob.m > 0 && ob.n > 0) {
+      const int64_t last = ob.c_off + (int64_t)(ob.m - 1) + (int64_t)(ob.n - 1) * ob.ldc;
+      TSC_REQUIRE(ob.c_off >= 0 && last < sz.size[out_buf], "output block outside its buffer");
+    }
+    for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) {
+      const tsc_contrib &t = c[ci];
+      TSC_REQUIRE(t.a_op >= 0 && t.a_op <= TSC_OP_I && t.b_op >= 0 && t.b_op <= TSC_OP_I, "bad op code");
+      TSC_REQUIRE(!(t.a_op == TSC_OP_I && t.b_op == TSC_OP_I), "identity times identity");
+      // stored shape of each operand: op(A) is m x k (or m x n when B = I)
+      auto check_operand = [&](int buf, int64_t off, int ld, int op, int rows, int cols) -> int {
+        if (op == TSC_OP_I) return TSC_OK;
+        TSC_REQUIRE(buf >= 0 && buf < 4 && readable[buf], "operand reads a buffer this stage may not read");
+        const bool tr = (op == TSC_OP_T || op == TSC_OP_C);
+        const int sr = tr ? cols : rows, sc = tr ? rows : cols;
+        if (sr == 0 || sc == 0) return TSC_OK;
+        TSC_REQUIRE(ld >= std::max(1, sr), "operand leading dimension");
+        const int64_t last = off + (int64_t)(sr - 1) + (int64_t)(sc - 1) * ld;
+        TSC_REQUIRE(off >= 0 && last < sz.size[buf], "operand outside its buffer");
+        return TSC_OK;
+      };
+      int rc;
+      if (t.b_op == TSC_OP_I) {
+        rc = check_operand(t.a_buf, t.a_off, t.lda, t.a_op, ob.m, ob.n);
+        if (rc) return rc;
+        *flops += 8.0 * ob.m * ob.n;
+      } else if (t.a_op == TSC_OP_I) {
+        rc = check_operand(t.b_buf, t.b_off, t.ldb, t.b_op, ob.m, ob.n);
+        if (rc) return rc;
+        *flops += 8.0 * ob.m * ob.n;
+      } else {
+        TSC_REQUIRE(t.k >= 0, "negative k");
+        rc = check_operand(t.a_buf, t.a_off, t.lda, t.a_op, ob.m, t.k);
+        if (rc) return rc;
+        rc = check_operand(t.b_buf, t.b_off, t.ldb, t.b_op, t.k, ob.n);
+        if (rc) return rc;
+        *flops += 8.0 * ob.m * ob.n * (double)t.k;
+      }
+    }
+  }
+  return TSC_OK;
+}
+
+std::vector<Tile> make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no) {
+  std::vector<Tile> tiles;
+  std::vector<double> cost;
+  for (int32_t b = 0; b < no; ++b) {
+    const tsc_outblock &ob = o[b];
+    if (ob.m == 0 || ob.n == 0) continue;
+    double w = 0.0;
+    for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci)
+      w += (c[ci].a_op == TSC_OP_I || c[ci].b_op == TSC_OP_I) ? 1.0 : (double)std::max(c[ci].k, 1);
+    w = std::max(w, 1.0);
+    const int nti = (ob.m + kTM - 1) / kTM, ntj = (ob.n + kTN - 1) / kTN;
+    for (int ti = 0; ti < nti; ++ti)
+      for (int tj = 0; tj < ntj; ++tj) {
+        tiles.push_back(Tile{b, (int16_t)ti, (int16_t)tj});
+        cost.push_back(w);
+      }
+  }
+  std::vector<size_t> order(tiles.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<Tile> sorted(tiles.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
+  return sorted;
+}
+
+template <typename T>
+int upload(T **dst, const T *src, size_t n, hipStream_t s) {
+  if (n == 0) {
+    *dst = nullptr;
+    return TSC_OK;
+  }
+  TSC_HIP_CHECK(hipMalloc((void **)dst, sizeof(T) * n));
+  TSC_HIP_CHECK(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+  return TSC_OK;
+}
+
+}  // namespace
+
+extern "C" int tsc_plan_destroy(tsc_plan *p) {
+  if (!p) return TSC_OK;
+  hipSetDevice(p->ctx->device);
+  hipStreamSynchronize(p->ctx->stream);
+  if (p->own_ops) hipFree(p->d_ops64);
+  hipFree(p->d_ops32);
+  hipFree(p->d_mid64);
+  hipFree(p->d_mid32);
+  for (int s = 0; s < 2; ++s) {
+    hipFree(p->d_c[s]);
+    hipFree(p->d_o[s]);
+    hipFree(p->d_t[s]);
+  }
+  for (int i = 0; i < 3; ++i) {
+    if (p->ev[i]) hipEventDestroy(p->ev[i]);
+    if (p->gexec[i]) hipGraphExecDestroy(p->gexec[i]);
+  }
+  delete p;
+  return TSC_OK;
+}
+
+namespace tsc {
+int plan_create_impl(tsc_ctx *ctx, const double *ops, const double2 *ops_dev, int64_t ops_elems, int64_t mid_elems,
+                     const tsc_contrib *c1, int32_t nc1, const tsc_outblock *o1, int32_t no1, const tsc_contrib *c2,
+                     int32_t nc2, const tsc_outblock *o2, int32_t no2, int64_t in_elems, int64_t out_elems,
+                     tsc_plan **out) {
+  TSC_REQUIRE(ctx && out, "tsc_plan_create: null ctx/out");
+  TSC_REQUIRE(ops_elems >= 0 && mid_elems >= 0 && in_elems >= 0 && out_elems >= 0, "negative sizes");
+  TSC_REQUIRE(nc1 >= 0 && no1 >= 0 && nc2 >= 0 && no2 >= 0, "negative counts");
+
+  StageCheck sz{{ops_elems, in_elems, mid_elems, out_elems}};
+  double flops = 0.0;
+  {
+    const int rd1[4] = {1, 1, 0, 0};
+    int rc = check_stage(c1, nc1, o1, no1, sz, TSC_BUF_MID, rd1, &flops);
+    if (rc) return rc;
+    const int rd2[4] = {1, 1, 1, 0};
+    rc = check_stage(c2, nc2, o2, no2, sz, TSC_BUF_OUT, rd2, &flops);
+    if (rc) return rc;
+  }
+  TSC_HIP_CHECK(hipSetDevice(ctx->device));
+  tsc_plan *p = new tsc_plan();
+  p->ctx = ctx;
+  p->ops_elems = ops_elems;
+  p->mid_elems = mid_elems;
+  p->in_elems = in_elems;
+  p->out_elems = out_elems;
+  p->flops = flops;
+  int rc = TSC_OK;
+  auto fail = [&](int r) {
+    tsc_plan_destroy(p);
+    return r;
+  };
+  if (ops_dev) {
+    p->d_ops64 = const_cast<double2 *>(ops_dev);
+    p->own_ops = false;
+  } else if (ops) {
+    if ((rc = upload<double2>(&p->d_ops64, reinterpret_cast<const double2 *>(ops), ops_elems, ctx->stream)))
+      return fail(rc);
+  } else if (ops_elems > 0) {
+    hipError_t e = hipMalloc((void **)&p->d_ops64, sizeof(double2) * ops_elems);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc(ops arena) failed: ") + hipGetErrorString(e));
+      return fail(TSC_ENOMEM);
+    }
+    e = hipMemsetAsync(p->d_ops64, 0, sizeof(double2) * ops_elems, ctx->stream);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMemsetAsync(ops arena) failed: ") + hipGetErrorString(e));
+      return fail(TSC_EHIP);
+    }
+  }
+  const tsc_contrib *cs[2] = {c1, c2};
+  const tsc_outblock *os[2] = {o1, o2};
+  const int32_t ncs[2] = {nc1, nc2}, nos[2] = {no1, no2};
+  for (int s = 0; s < 2; ++s) {
+    if ((rc = upload<tsc_contrib>(&p->d_c[s], cs[s], ncs[s], ctx->stream))) return fail(rc);
+    if ((rc = upload<tsc_outblock>(&p->d_o[s], os[s], nos[s], ctx->stream))) return fail(rc);
+    std::vector<Tile> tiles = make_tiles(cs[s], os[s], nos[s]);
+    p->nt[s] = (int)tiles.size();
+    if ((rc = upload<Tile>(&p->d_t[s], tiles.data(), tiles.size(), ctx->stream))) return fail(rc);
+    TSC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  for (int i = 0; i < 3; ++i) TSC_HIP_CHECK(hipEventCreate(&p->ev[i]));
+  *out = p;
+  return TSC_OK;
+}
+}  // namespace tsc
+
+extern "C" int tsc_plan_create(tsc_ctx *ctx, const double *ops, int64_t ops_elems, int64_t mid_elems,
+                               const tsc_contrib *c1, int32_t nc1, const tsc_outblock *o1, int32_t no1,
+                               const tsc_contrib *c2, int32_t nc2, const tsc_outblock *o2, int32_t no2,
+                               int64_t in_elems, int64_t out_elems, tsc_plan **out) {
+  return plan_create_impl(ctx, ops, nullptr, ops_elems, mid_elems, c1, nc1, o1, no1, c2, nc2, o2, no2, in_elems,
+                          out_elems, out);
+}
+
+extern "C" int tsc_plan_update_ops(tsc_plan *p, const double *ops) {
+  TSC_REQUIRE(p && (ops || p->ops_elems == 0), "tsc_plan_update_ops arguments");
+  if (p->ops_elems == 0) return TSC_OK;
+  TSC_HIP_CHECK(hipMemcpyAsync(p->d_ops64, ops, sizeof(double2) * p->ops_elems, hipMemcpyHostToDevice, p->ctx->stream));
+  if (p->d_ops32) {
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(4096), dim3(256), 0, p->ctx->stream, p->d_ops64, p->d_ops32,
+                       p->ops_elems);
+    TSC_HIP_CHECK(hipGetLastError());
+  }
+  TSC_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  return TSC_OK;
+}
+
+extern "C" int tsc_plan_fill_ops_gaussian(tsc_plan *p, int64_t off, int64_t count, uint64_t seed, uint64_t key,
+                                          double sigma) {
+  TSC_REQUIRE(p && off >= 0 && count >= 0 && off + count <= p->ops_elems, "tsc_plan_fill_ops_gaussian range");
+  TSC_REQUIRE(p->own_ops, "tsc_plan_fill_ops_gaussian: borrowed arena");
+  int rc = fill_gaussian(p->ctx, p->d_ops64 + off, count, seed, key, sigma);
+  if (rc) return rc;
+  if (p->d_ops32) {
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(4096), dim3(256), 0, p->ctx->stream, p->d_ops64 + off,
+                       p->d_ops32 + off, count);
+    TSC_HIP_CHECK(hipGetLastError());
+  }
+  TSC_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  return TSC_OK;
+}
+
+extern "C" int tsc_plan_flops(const tsc_plan *p, double *flops) {
+  TSC_REQUIRE(p && flops, "tsc_plan_flops arguments");
+  *flops = p->flops;
+  return TSC_OK;
+}
+
+namespace {
+int ensure_precision(tsc_plan *p, int prec) {
+  hipStream_t s = p->ctx->stream;
+  if (prec == TSC_PREC_F64) {
+    if (!p->d_mid64 && p->mid_elems > 0) TSC_HIP_CHECK(hipMalloc((void **)&p->d_mid64, sizeof(double2) * p->mid_elems));
+  } else {
+    if (!p->d_ops32 && p->ops_elems > 0) {
+      TSC_HIP_CHECK(hipMalloc((void **)&p->d_ops32, sizeof(float2) * p->ops_elems));
+      hipLaunchKernelGGL(f64_to_f32_kernel, dim3(4096), dim3(256), 0, s, p->d_ops64, p->d_ops32, p->ops_elems);
+      TSC_HIP_CHECK(hipGetLastError());
+    }
+    if (!p->d_mid32 && p->mid_elems > 0) TSC_HIP_CHECK(hipMalloc((void **)&p->d_mid32, sizeof(float2) * p->mid_elems));
+  }
+  return TSC_OK;
+}
+
+int enqueue_hpsi(tsc_plan *p, const double2 *in, double2 *out, int prec, bool events) {
+  hipStream_t s = p->ctx->stream;
+  Bufs b;
+  b.p[TSC_BUF_OPS] = (prec == TSC_PREC_F64) ? (const void *)p->d_ops64 : (const void *)p->d_ops32;
+  b.p[TSC_BUF_IN] = in;
+  b.p[TSC_BUF_MID] = (prec == TSC_PREC_F64) ? (const void *)p->d_mid64 : (const void *)p->d_mid32;
+  b.p[TSC_BUF_OUT] = out;
+  if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[0], s));
+  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[0], p->d_c[0], p->d_t[0], p->nt[0], s));
+  if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[1], s));
+  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[1], p->d_c[1], p->d_t[1], p->nt[1], s));
+  if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[2], s));
+  return TSC_OK;
+}
+}  // namespace
+
+namespace tsc {
+int hpsi_dev(tsc_plan *p, const double2 *in, double2 *out, int prec) {
+  TSC_REQUIRE(prec >= TSC_PREC_F64 && prec <= TSC_PREC_BF16X3, "unknown precision");
+  int rc = ensure_precision(p, prec);
+  if (rc) return rc;
+  rc = enqueue_hpsi(p, in, out, prec, true);
+  if (rc) return rc;
+  p->timed = true;
+  return TSC_OK;
+}
+}  // namespace tsc
+
+extern "C" int tsc_hpsi(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int prec) {
+  TSC_REQUIRE(p && in && out, "tsc_hpsi arguments");
+  TSC_REQUIRE(in->n >= p->in_elems && out->n >= p->out_elems, "tsc_hpsi: vector shorter than the plan needs");
+  TSC_REQUIRE(in->d != out->d || p->in_elems == 0, "tsc_hpsi: in and out must differ");
+  return hpsi_dev(p, in->d, out->d, prec);
+}
+
+extern "C" int tsc_hpsi_host(tsc_plan *p, const double *in, double *out, int prec) {
+  TSC_REQUIRE(p && (in || p->in_elems == 0) && (out || p->out_elems == 0), "tsc_hpsi_host arguments");
+  tsc_vec *vi = nullptr, *vo = nullptr;
+  int rc = tsc_vec_create(p->ctx, p->in_elems, &vi);
+  if (rc) return rc;
+  rc = tsc_vec_create(p->ctx, p->out_elems, &vo);
+  if (!rc) rc = tsc_vec_upload(vi, in);
+  if (!rc) rc = tsc_hpsi(p, vi, vo, prec);
+  if (!rc) rc = tsc_vec_download(vo, out);
+  tsc_vec_destroy(vi);
+  tsc_vec_destroy(vo);
+  return rc;
+}
+
+extern "C" int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int prec) {
+  TSC_REQUIRE(p && in && out, "tsc_hpsi_graph arguments");
+  TSC_REQUIRE(prec >= TSC_PREC_F64 && prec <= TSC_PREC_BF16X3, "unknown precision");
+  TSC_REQUIRE(in->n >= p->in_elems && out->n >= p->out_elems && in->d != out->d, "tsc_hpsi_graph vectors");
+  int rc = ensure_precision(p, prec);
+  if (rc) return rc;
+  hipStream_t s = p->ctx->stream;
+  if (!p->gexec[prec] || p->g_in[prec] != in->d || p->g_out[prec] != out->d) {
+    if (p->gexec[prec]) {
+      TSC_HIP_CHECK(hipGraphExecDestroy(p->gexec[prec]));
+      p->gexec[prec] = nullptr;
+    }
+    TSC_HIP_CHECK(hipStreamSynchronize(s));
+    hipGraph_t g;
+    TSC_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_hpsi(p, in->d, out->d, prec, false);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+      return TSC_EHIP;
+    }
+    TSC_HIP_CHECK(hipGraphInstantiate(&p->gexec[prec], g, nullptr, nullptr, 0));
+    TSC_HIP_CHECK(hipGraphDestroy(g));
+    p->g_in[prec] = in->d;
+    p->g_out[prec] = out->d;
+  }
+  TSC_HIP_CHECK(hipEventRecord(p->ev[0], s));
+  TSC_HIP_CHECK(hipGraphLaunch(p->gexec[prec], s));
+  TSC_HIP_CHECK(hipEventRecord(p->ev[2], s));
+  p->timed = false;
+  return TSC_OK;
+}
+
+extern "C" int tsc_plan_last_times(const tsc_plan *p, float *ms1, float *ms2) {
+  TSC_REQUIRE(p && ms1 && ms2, "tsc_plan_last_times arguments");
+  TSC_HIP_CHECK(hipEventSynchronize(p->ev[2]));
+  if (p->timed) {
+    TSC_HIP_CHECK(hipEventElapsedTime(ms1, p->ev[0], p->ev[1]));
+    TSC_HIP_CHECK(hipEventElapsedTime(ms2, p->ev[1], p->ev[2]));
+  } else {
+    float t = 0.f;
+    TSC_HIP_CHECK(hipEventElapsedTime(&t, p->ev[0], p->ev[2]));
+    *ms1 = 0.f;
+    *ms2 = t;
+  }
+  return TSC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos with full re-orthogonalisation, following lanczos_memo_new
+// (hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/eigen.f90:4-199) for levels = 1:
+// up to max_outer+1 passes of at most max_inner Krylov vectors; inner
+// products keep the real part only (wave_product, block.f90:1092-1108);
+// classical Gram-Schmidt against every previous vector then normalise
+// (Gram_Schmidt_process, block.f90:1251-1289); convergence when the lowest
+// Ritz value moves by <= tol, or the residual norm beta <= tol.
+namespace {
+
+struct LanczosResult {
+  double energy;
+  int matvecs;
+  int restarts;
+};
+
+int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer, int prec, LanczosResult *res) {
+  tsc_ctx *ctx = p->ctx;
+  const int64_t n = psi->n;
+  TSC_REQUIRE(max_inner >= 1 && max_inner + 1 <= kMaxVecs, "max_inner out of range");
+  TSC_REQUIRE(n == p->in_elems && n == p->out_elems, "psi size does not match the plan");
+  std::vector<double2 *> V(max_inner + 1, nullptr);
+  for (auto &v : V) TSC_HIP_CHECK(hipMalloc((void **)&v, sizeof(double2) * std::max<int64_t>(n, 1)));
+  auto free_all = [&]() {
+    hipStreamSynchronize(ctx->stream);
+    for (auto v : V) hipFree(v);
+  };
+  std::vector<double> alpha(max_inner + 1, 0.0), beta(max_inner + 1, 0.0), z(max_inner + 1);
+  double lastmin = 1024.0;
+  int matvecs = 0, iter_out = 0, last = 1;
+  bool done = false;
+  int rc = TSC_OK;
+#define LZ_CALL(x)        \
+  do {                    \
+    rc = (x);             \
+    if (rc) {             \
+      free_all();         \
+      return rc;          \
+    }                     \
+  } while (0)
+  while (iter_out <= max_outer && !done) {
+    ++iter_out;
+    last = 1;
+    std::fill(alpha.begin(), alpha.end(), 0.0);
+    std::fill(beta.begin(), beta.end(), 0.0);
+    double ritzmin = lastmin;
+    // v1 = psi / |psi|
+    TSC_HIP_CHECK(hipMemcpyAsync(V[0], psi->d, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    {
+      double nn;
+      const double2 *vv[1] = {V[0]};
+      LZ_CALL(dots_re(ctx, vv, 1, V[0], n, &nn));
+      LZ_CALL(scale(ctx, V[0], 1.0 / std::sqrt(nn), n));
+    }
+    LZ_CALL(hpsi_dev(p, V[0], V[1], prec));
+    ++matvecs;
+    {
+      const double2 *vv[1] = {V[0]};
+      LZ_CALL(dots_re(ctx, vv, 1, V[1], n, &alpha[1]));
+    }
+    ritzmin = alpha[1];
+    bool converged_inner = false;
+    while (last < max_inner) {
+      // w = H v_last - alpha v_last - beta v_{last-1}   (w lives in V[last])
+      {
+        const double2 *vv[2] = {V[last - 1], last >= 2 ? V[last - 2] : nullptr};
+        const double cc[2] = {-alpha[last], last >= 2 ? -beta[last - 1] : 0.0};
+        LZ_CALL(combine(ctx, V[last], 1.0, vv, cc, last >= 2 ? 2 : 1, n));
+      }
+      {
+        double nn;
+        const double2 *vv[1] = {V[last]};
+        LZ_CALL(dots_re(ctx, vv, 1, V[last], n, &nn));
+        beta[last] = std::sqrt(nn);
+      }
+      if (std::fabs(beta[last]) <= tol) {
+        converged_inner = true;
+        done = true;
+        break;
+      }
+      LZ_CALL(scale(ctx, V[last], 1.0 / beta[last], n));
+      // classical Gram-Schmidt with real overlaps, then normalise
+      {
+        std::vector<const double2 *> vv(V.begin(), V.begin() + last);
+        std::vector<double> uv(last), uu(last), cc(last);
+        LZ_CALL(dots_re(ctx, vv.data(), last, V[last], n, uv.data()));
+        for (int j = 0; j < last; ++j) {
+          const double2 *one[1] = {V[j]};
+          LZ_CALL(dots_re(ctx, one, 1, V[j], n, &uu[j]));
+          cc[j] = -uv[j] / uu[j];
+        }
+        LZ_CALL(combine(ctx, V[last], 1.0, vv.data(), cc.data(), last, n));
+        double nn;
+        const double2 *me[1] = {V[last]};
+        LZ_CALL(dots_re(ctx, me, 1, V[last], n, &nn));
+        LZ_CALL(scale(ctx, V[last], 1.0 / std::sqrt(nn), n));
+      }
+      ++last;
+      LZ_CALL(hpsi_dev(p, V[last - 1], V[last], prec));
+      ++matvecs;
+      {
+        const double2 *vv[1] = {V[last - 1]};
+        LZ_CALL(dots_re(ctx, vv, 1, V[last], n, &alpha[last]));
+      }
+      const double e0 = tridiag_lowest(&alpha[1], &beta[1], last);
+      if (std::fabs(e0 - ritzmin) <= tol || (last == max_inner && iter_out == max_outer)) {
+        lastmin = e0;
+        converged_inner = true;
+        done = true;
+        break;
+      }
+      ritzmin = e0;
+      lastmin = ritzmin;
+    }
+    // Ritz vector of the current Krylov basis -> psi (restart or final)
+    const double e0 = tridiag_lowest(&alpha[1], &beta[1], last);
+    tridiag_vector(&alpha[1], &beta[1], last, e0, z.data());
+    {
+      std::vector<const double2 *> vv(V.begin(), V.begin() + last);
+      LZ_CALL(combine(ctx, psi->d, 0.0, vv.data(), z.data(), last, n));
+      double nn;
+      const double2 *me[1] = {psi->d};
+      LZ_CALL(dots_re(ctx, me, 1, psi->d, n, &nn));
+      LZ_CALL(scale(ctx, psi->d, 1.0 / std::sqrt(nn), n));
+    }
+    lastmin = e0;
+    res->energy = e0;
+    (void)converged_inner;
+  }
+#undef LZ_CALL
+  TSC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  free_all();
+  res->matvecs = matvecs;
+  res->restarts = iter_out - 1;
+  return TSC_OK;
+}
+
+}  // namespace
+
+extern "C" int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer, int prec,
+                           int refine_iters, double *energy, int32_t *info) {
+  TSC_REQUIRE(p && psi && energy, "tsc_lanczos arguments");
+  TSC_REQUIRE(max_outer >= 0 && refine_iters >= 0, "tsc_lanczos iteration counts");
+  LanczosResult r{0.0, 0, 0};
+  int rc = lanczos_pass(p, psi, tol, max_inner, max_outer, prec, &r);
+  if (rc) return rc;
+  int mv = r.matvecs, rs = r.restarts;
+  if (prec != TSC_PREC_F64 && refine_iters > 0) {
+    LanczosResult r2{0.0, 0, 0};
+    rc = lanczos_pass(p, psi, tol, std::max(2, refine_iters), 0, TSC_PREC_F64, &r2);
+    if (rc) return rc;
+    r.energy = r2.energy;
+    mv += r2.matvecs;
+  }
+  *energy = r.energy;
+  if (info) {
+    info[0] = mv;
+    info[1] = rs;
+  }
+  return TSC_OK;
+}

@@ -1,0 +1,440 @@
+// Grouped complex contraction kernels for gfx950 (CDNA4).
+//
+// One workgroup (4 waves, 256 lanes) owns one 64x64 tile of one output
+// block and accumulates every contribution of that block in registers:
+//     C_tile = sum_c coef_c * op_a(A_c) * op_b(B_c)
+// so no atomics are needed and every output element is written once.
+// This is the batched replacement for the per-sector ZGEMM pairs of
+// model_wave's kernels (hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/get_sys.f90:
+// 1817, 1869, 2555-2560, 3385-3408, ...).
+//
+// Waves are arranged 2x2 over the tile; each wave computes a 32x32 sub-tile
+// as 2x2 MFMA 16x16 blocks, real and imaginary parts in separate
+// accumulators (4 real MFMAs per complex k-step).
+//   FP64  : v_mfma_f64_16x16x4f64, TK = 16
+//   FP32  : v_mfma_f32_16x16x4f32, TK = 16 (complex FP32 products)
+//   BF16x3: v_mfma_f32_16x16x32_bf16 on hi/lo splits, TK = 32
+//           a*b ~ ah*bh + ah*bl + al*bh, FP32 accumulation.
+#include "tsc_internal.h"
+
+namespace tsc {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int TM = kTM, TN = kTN;
+
+// ---------------------------------------------------------------------
+// element loads: buffers IN/OUT are complex128, OPS/MID complex<CT>
+template <typename CT>
+__device__ __forceinline__ void ldbuf(const Bufs &B, int buf, int64_t off, CT &re, CT &im) {
+  if (buf == TSC_BUF_IN || buf == TSC_BUF_OUT || sizeof(CT) == 8) {
+    double2 v = reinterpret_cast<const double2 *>(B.p[buf])[off];
+    re = (CT)v.x;
+    im = (CT)v.y;
+  } else {
+    float2 v = reinterpret_cast<const float2 *>(B.p[buf])[off];
+    re = (CT)v.x;
+    im = (CT)v.y;
+  }
+}
+
+// op(X)[r, c] for a column-major X with leading dimension ld
+template <typename CT>
+__device__ __forceinline__ void ld_op(const Bufs &B, int buf, int64_t off, int ld, int op, int r,
+                                      int c, CT &re, CT &im) {
+  const bool tr = (op == TSC_OP_T || op == TSC_OP_C);
+  const int64_t idx = tr ? (off + (int64_t)c + (int64_t)r * ld) : (off + (int64_t)r + (int64_t)c * ld);
+  ldbuf<CT>(B, buf, idx, re, im);
+  if (op == TSC_OP_C || op == TSC_OP_CONJ) im = -im;
+}
+
+template <typename CT>
+__device__ __forceinline__ void cmul(CT ar, CT ai, CT br, CT bi, CT &cr, CT &ci) {
+  cr = ar * br - ai * bi;
+  ci = ar * bi + ai * br;
+}
+
+// ---------------------------------------------------------------------
+// FP64 / FP32 kernel
+template <typename CT>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<double> {
+  typedef d4 acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaTraits<float> {
+  typedef f4 acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename CT, int TK>
+__global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outblock *__restrict__ blocks,
+                                                      const tsc_contrib *__restrict__ contribs,
+                                                      const Tile *__restrict__ tiles) {
+  typedef typename MfmaTraits<CT>::acc_t acc_t;
+  constexpr int PAD = (sizeof(CT) == 8) ? 16 : 32;  // row padding (elements)
+  __shared__ CT As_r[TK][TM + PAD], As_i[TK][TM + PAD];
+  __shared__ CT Bs_r[TK][TN + PAD], Bs_i[TK][TN + PAD];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave & 1) * 32;
+  const int wn = (wave >> 1) * 32;
+  const Tile t = tiles[blockIdx.x];
+  const tsc_outblock ob = blocks[t.blk];
+  const int i0 = t.ti * TM, j0 = t.tj * TN;
+  const int m = ob.m, n = ob.n;
+
+  acc_t accR[2][2], accI[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      accR[a][b] = acc_t{0, 0, 0, 0};
+      accI[a][b] = acc_t{0, 0, 0, 0};
+    }
+
+  for (int ci = ob.cbeg; ci < ob.cend; ++ci) {
+    const tsc_contrib c = contribs[ci];
+    const CT cre = (CT)c.coef_re, cim = (CT)c.coef_im;
+    if (c.a_op == TSC_OP_I || c.b_op == TSC_OP_I) {
+      // elementwise term: acc(i,j) += coef * X(i,j)
+      const bool useA = (c.b_op == TSC_OP_I);
+      const int buf = useA ? c.a_buf : c.b_buf;
+      const int64_t off = useA ? c.a_off : c.b_off;
+      const int ldx = useA ? c.lda : c.ldb;
+      const int op = useA ? c.a_op : c.b_op;
+      if (op == TSC_OP_I) continue;  // I*I is not a valid term
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+            const int j = j0 + wn + b * 16 + (lane & 15);
+            if (i < m && j < n) {
+              CT xr, xi, pr, pi;
+              ld_op<CT>(bufs, buf, off, ldx, op, i, j, xr, xi);
+              cmul<CT>(cre, cim, xr, xi, pr, pi);
+              accR[a][b][r] += pr;
+              accI[a][b][r] += pi;
+            }
+          }
+      continue;
+    }
+    const int K = c.k;
+    const bool a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
+    const bool b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
+    for (int k0 = 0; k0 < K; k0 += TK) {
+      // ---- stage A tile (TM x TK) scaled by coef
+#pragma unroll
+      for (int r = 0; r < (TM * TK) / 256; ++r) {
+        int ii, kk;
+        if (!a_tr) {
+          ii = tid % TM;
+          kk = tid / TM + r * (256 / TM);
+        } else {
+          kk = tid % TK;
+          ii = tid / TK + r * (256 / TK);
+        }
+        const int gi = i0 + ii, gk = k0 + kk;
+        CT xr = 0, xi = 0;
+        if (gi < m && gk < K) ld_op<CT>(bufs, c.a_buf, c.a_off, c.lda, c.a_op, gi, gk, xr, xi);
+        CT pr, pi;
+        cmul<CT>(cre, cim, xr, xi, pr, pi);
+        As_r[kk][ii] = pr;
+        As_i[kk][ii] = pi;
+      }
+      // ---- stage B tile (TK x TN)
+#pragma unroll
+      for (int r = 0; r < (TN * TK) / 256; ++r) {
+        int jj, kk;
+        if (b_tr) {
+          jj = tid % TN;
+          kk = tid / TN + r * (256 / TN);
+        } else {
+          kk = tid % TK;
+          jj = tid / TK + r * (256 / TK);
+        }
+        const int gj = j0 + jj, gk = k0 + kk;
+        CT xr = 0, xi = 0;
+        if (gj < n && gk < K) ld_op<CT>(bufs, c.b_buf, c.b_off, c.ldb, c.b_op, gk, gj, xr, xi);
+        Bs_r[kk][jj] = xr;
+        Bs_i[kk][jj] = xi;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < TK / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        CT ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          ar[a] = As_r[kr][wm + a * 16 + (lane & 15)];
+          ai[a] = As_i[kr][wm + a * 16 + (lane & 15)];
+          nai[a] = -ai[a];
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          br[b] = Bs_r[kr][wn + b * 16 + (lane & 15)];
+          bi[b] = Bs_i[kr][wn + b * 16 + (lane & 15)];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            accR[a][b] = MfmaTraits<CT>::mma(ar[a], br[b], accR[a][b]);
+            accR[a][b] = MfmaTraits<CT>::mma(nai[a], bi[b], accR[a][b]);
+            accI[a][b] = MfmaTraits<CT>::mma(ar[a], bi[b], accI[a][b]);
+            accI[a][b] = MfmaTraits<CT>::mma(ai[a], br[b], accI[a][b]);
+          }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- write the tile
+  const bool out_f64 = (ob.c_buf == TSC_BUF_IN || ob.c_buf == TSC_BUF_OUT || sizeof(CT) == 8);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+        const int j = j0 + wn + b * 16 + (lane & 15);
+        if (i < m && j < n) {
+          const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
+          if (out_f64) {
+            double2 *C = reinterpret_cast<double2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+            double2 v = make_double2((double)accR[a][b][r], (double)accI[a][b][r]);
+            if (ob.mode) {
+              const double2 o = C[idx];
+              v.x += o.x;
+              v.y += o.y;
+            }
+            C[idx] = v;
+          } else {
+            float2 *C = reinterpret_cast<float2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+            float2 v = make_float2((float)accR[a][b][r], (float)accI[a][b][r]);
+            if (ob.mode) {
+              const float2 o = C[idx];
+              v.x += o.x;
+              v.y += o.y;
+            }
+            C[idx] = v;
+          }
+        }
+      }
+}
+
+// ---------------------------------------------------------------------
+// BF16x3 kernel: operands are loaded as FP32, split into bf16 hi/lo in LDS.
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+__global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const tsc_outblock *__restrict__ blocks,
+                                                             const tsc_contrib *__restrict__ contribs,
+                                                             const Tile *__restrict__ tiles) {
+  constexpr int TK = 32;
+  // layout [row][k] so that each lane reads 8 consecutive k (16 bytes)
+  constexpr int KP = TK + 8;
+  __shared__ __bf16 Ah_r[TM][KP], Al_r[TM][KP], Ah_i[TM][KP], Al_i[TM][KP];
+  __shared__ __bf16 Bh_r[TN][KP], Bl_r[TN][KP], Bh_i[TN][KP], Bl_i[TN][KP];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave & 1) * 32;
+  const int wn = (wave >> 1) * 32;
+  const Tile t = tiles[blockIdx.x];
+  const tsc_outblock ob = blocks[t.blk];
+  const int i0 = t.ti * TM, j0 = t.tj * TN;
+  const int m = ob.m, n = ob.n;
+
+  f4 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      accR[a][b] = f4{0, 0, 0, 0};
+      accI[a][b] = f4{0, 0, 0, 0};
+    }
+
+  for (int ci = ob.cbeg; ci < ob.cend; ++ci) {
+    const tsc_contrib c = contribs[ci];
+    const float cre = (float)c.coef_re, cim = (float)c.coef_im;
+    if (c.a_op == TSC_OP_I || c.b_op == TSC_OP_I) {
+      const bool useA = (c.b_op == TSC_OP_I);
+      const int buf = useA ? c.a_buf : c.b_buf;
+      const int64_t off = useA ? c.a_off : c.b_off;
+      const int ldx = useA ? c.lda : c.ldb;
+      const int op = useA ? c.a_op : c.b_op;
+      if (op == TSC_OP_I) continue;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+            const int j = j0 + wn + b * 16 + (lane & 15);
+            if (i < m && j < n) {
+              float xr, xi, pr, pi;
+              ld_op<float>(bufs, buf, off, ldx, op, i, j, xr, xi);
+              cmul<float>(cre, cim, xr, xi, pr, pi);
+              accR[a][b][r] += pr;
+              accI[a][b][r] += pi;
+            }
+          }
+      continue;
+    }
+    const int K = c.k;
+    const bool a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
+    const bool b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
+    for (int k0 = 0; k0 < K; k0 += TK) {
+#pragma unroll
+      for (int r = 0; r < (TM * TK) / 256; ++r) {
+        int ii, kk;
+        if (!a_tr) {
+          ii = tid % TM;
+          kk = tid / TM + r * (256 / TM);
+        } else {
+          kk = tid % TK;
+          ii = tid / TK + r * (256 / TK);
+        }
+        const int gi = i0 + ii, gk = k0 + kk;
+        float xr = 0, xi = 0;
+        if (gi < m && gk < K) ld_op<float>(bufs, c.a_buf, c.a_off, c.lda, c.a_op, gi, gk, xr, xi);
+        float pr, pi;
+        cmul<float>(cre, cim, xr, xi, pr, pi);
+        split_bf16(pr, Ah_r[ii][kk], Al_r[ii][kk]);
+        split_bf16(pi, Ah_i[ii][kk], Al_i[ii][kk]);
+      }
+#pragma unroll
+      for (int r = 0; r < (TN * TK) / 256; ++r) {
+        int jj, kk;
+        if (b_tr) {
+          jj = tid % TN;
+          kk = tid / TN + r * (256 / TN);
+        } else {
+          kk = tid % TK;
+          jj = tid / TK + r * (256 / TK);
+        }
+        const int gj = j0 + jj, gk = k0 + kk;
+        float xr = 0, xi = 0;
+        if (gj < n && gk < K) ld_op<float>(bufs, c.b_buf, c.b_off, c.ldb, c.b_op, gk, gj, xr, xi);
+        split_bf16(xr, Bh_r[jj][kk], Bl_r[jj][kk]);
+        split_bf16(xi, Bh_i[jj][kk], Bl_i[jj][kk]);
+      }
+      __syncthreads();
+      // 16x16x32 bf16: lane holds row (lane&15), k = 8*(lane>>4) + 0..7
+      const int kq = 8 * (lane >> 4);
+      bf8 ahr[2], alr[2], ahi[2], ali[2], nahi[2], nali[2], bhr[2], blr[2], bhi[2], bli[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int row = wm + a * 16 + (lane & 15);
+        ahr[a] = *reinterpret_cast<const bf8 *>(&Ah_r[row][kq]);
+        alr[a] = *reinterpret_cast<const bf8 *>(&Al_r[row][kq]);
+        ahi[a] = *reinterpret_cast<const bf8 *>(&Ah_i[row][kq]);
+        ali[a] = *reinterpret_cast<const bf8 *>(&Al_i[row][kq]);
+        nahi[a] = -ahi[a];
+        nali[a] = -ali[a];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int col = wn + b * 16 + (lane & 15);
+        bhr[b] = *reinterpret_cast<const bf8 *>(&Bh_r[col][kq]);
+        blr[b] = *reinterpret_cast<const bf8 *>(&Bl_r[col][kq]);
+        bhi[b] = *reinterpret_cast<const bf8 *>(&Bh_i[col][kq]);
+        bli[b] = *reinterpret_cast<const bf8 *>(&Bl_i[col][kq]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          f4 r = accR[a][b], i = accI[a][b];
+          // Re += ar*br - ai*bi  (3-term split each)
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhr[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], blr[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nali[a], bhi[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bli[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhr[b], r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bhi[b], r, 0, 0, 0);
+          // Im += ar*bi + ai*br
+          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhi[b], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bli[b], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ali[a], bhr[b], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], blr[b], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhi[b], i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], bhr[b], i, 0, 0, 0);
+          accR[a][b] = r;
+          accI[a][b] = i;
+        }
+      __syncthreads();
+    }
+  }
+
+  const bool out_f64 = (ob.c_buf == TSC_BUF_IN || ob.c_buf == TSC_BUF_OUT);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+        const int j = j0 + wn + b * 16 + (lane & 15);
+        if (i < m && j < n) {
+          const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
+          if (out_f64) {
+            double2 *C = reinterpret_cast<double2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+            double2 v = make_double2((double)accR[a][b][r], (double)accI[a][b][r]);
+            if (ob.mode) {
+              const double2 o = C[idx];
+              v.x += o.x;
+              v.y += o.y;
+            }
+            C[idx] = v;
+          } else {
+            float2 *C = reinterpret_cast<float2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+            float2 v = make_float2(accR[a][b][r], accI[a][b][r]);
+            if (ob.mode) {
+              const float2 o = C[idx];
+              v.x += o.x;
+              v.y += o.y;
+            }
+            C[idx] = v;
+          }
+        }
+      }
+}
+
+hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_blocks,
+                          const tsc_contrib *d_contribs, const Tile *d_tiles, int ntiles,
+                          hipStream_t stream) {
+  if (ntiles <= 0) return hipSuccess;
+  dim3 grid(ntiles), block(256);
+  if (prec == TSC_PREC_F64) {
+    hipLaunchKernelGGL((grouped_kernel<double, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs,
+                       d_tiles);
+  } else if (prec == TSC_PREC_F32) {
+    hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs,
+                       d_tiles);
+  } else {
+    hipLaunchKernelGGL(grouped_kernel_bf16x3, grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tsc

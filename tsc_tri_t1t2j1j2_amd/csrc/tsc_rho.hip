@@ -1,0 +1,399 @@
+// Reduced density matrices, their Hermitian eigensolve, the truncation rule
+// and the U^H O U renormalisation, on the grouped-contraction engine.
+//
+// Follows hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/den_mat.f90:4-99 (rho per
+// sector), :184-418 (Get_truncation_operator) and block.f90:3465-3683
+// (update_trun_dia / update_trun_ndia).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "tsc_internal.h"
+
+using namespace tsc;
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+};
+
+int ensure_blas(tsc_ctx *ctx, rocblas_handle *h) {
+  static thread_local rocblas_handle handle = nullptr;
+  static thread_local int handle_dev = -1;
+  if (!handle || handle_dev != ctx->device) {
+    if (handle) rocblas_destroy_handle(handle);
+    if (rocblas_create_handle(&handle) != rocblas_status_success) {
+      set_error("rocblas_create_handle failed");
+      handle = nullptr;
+      return TSC_EHIP;
+    }
+    handle_dev = ctx->device;
+  }
+  rocblas_set_stream(handle, ctx->stream);
+  *h = handle;
+  return TSC_OK;
+}
+
+// rho_s for every sector, into d_rho (sum d_s^2), through a stage-2-only plan
+int build_rho(tsc_ctx *ctx, const double2 *d_psi, int64_t psi_elems, int32_t nsec, const int32_t *sys_dim,
+              const int32_t *env_dim, int side, double2 *d_rho, int64_t rho_elems, std::vector<int64_t> &rho_off) {
+  std::vector<tsc_contrib> c(nsec);
+  std::vector<tsc_outblock> o(nsec);
+  int64_t poff = 0, roff = 0;
+  rho_off.assign(nsec, 0);
+  for (int32_t s = 0; s < nsec; ++s) {
+    const int sd = sys_dim[s], ed = env_dim[s];
+    TSC_REQUIRE(sd >= 0 && ed >= 0, "negative sector dimension");
+    const int d = side == 0 ? sd : ed;
+    rho_off[s] = roff;
+    tsc_contrib &t = c[s];
+    std::memset(&t, 0, sizeof(t));
+    t.a_off = poff;
+    t.b_off = poff;
+    t.lda = std::max(1, sd);
+    t.ldb = std::max(1, sd);
+    t.a_buf = TSC_BUF_IN;
+    t.b_buf = TSC_BUF_IN;
+    if (side == 0) {  // psi psi^H : (sd x ed)(ed x sd)
+      t.a_op = TSC_OP_N;
+      t.b_op = TSC_OP_C;
+      t.k = ed;
+    } else {  // psi^H psi : (ed x sd)(sd x ed)
+      t.a_op = TSC_OP_C;
+      t.b_op = TSC_OP_N;
+      t.k = sd;
+    }
+    t.coef_re = 1.0;
+    tsc_outblock &b = o[s];
+    b.c_off = roff;
+    b.ldc = std::max(1, d);
+    b.m = d;
+    b.n = d;
+    b.c_buf = TSC_BUF_OUT;
+    b.cbeg = s;
+    b.cend = s + 1;
+    b.mode = 0;
+    poff += (int64_t)sd * ed;
+    roff += (int64_t)d * d;
+  }
+  TSC_REQUIRE(poff <= psi_elems, "sector dimensions exceed psi length");
+  TSC_REQUIRE(roff <= rho_elems, "rho buffer too small");
+  tsc_plan *p = nullptr;
+  int rc = plan_create_impl(ctx, nullptr, nullptr, 0, 0, nullptr, 0, nullptr, 0, c.data(), nsec, o.data(), nsec,
+                            psi_elems, rho_elems, &p);
+  if (rc) return rc;
+  rc = hpsi_dev(p, d_psi, d_rho, TSC_PREC_F64);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      set_error(std::string("rho build: ") + hipGetErrorString(e));
+      rc = TSC_EHIP;
+    }
+  }
+  tsc_plan_destroy(p);
+  return rc;
+}
+
+// in-place ascending eigensolve of every rho_s (ZHEEV 'V','U' role,
+// den_mat.f90:229-253); eigenvalues to d_w (sum d_s)
+int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, const std::vector<int64_t> &rho_off,
+                double *d_w, const std::vector<int64_t> &w_off) {
+  rocblas_handle h;
+  int rc = ensure_blas(ctx, &h);
+  if (rc) return rc;
+  int maxd = 1;
+  for (int32_t s = 0; s < nsec; ++s) maxd = std::max(maxd, dim[s]);
+  DevBuf e, info;
+  TSC_HIP_CHECK(hipMalloc(&e.p, sizeof(double) * maxd));
+  TSC_HIP_CHECK(hipMalloc(&info.p, sizeof(rocblas_int)));
+  for (int32_t s = 0; s < nsec; ++s) {
+    const int d = dim[s];
+    if (d == 0) continue;
+    rocblas_status st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, d,
+                                         reinterpret_cast<rocblas_double_complex *>(d_rho + rho_off[s]), d,
+                                         d_w + w_off[s], (double *)e.p, (rocblas_int *)info.p);
+    if (st != rocblas_status_success) {
+      set_error("rocsolver_zheevd failed for sector " + std::to_string(s));
+      return TSC_EHIP;
+    }
+  }
+  TSC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return TSC_OK;
+}
+
+// The reference's descending quicksort with its index array
+// (QuickSort_Double, block.f90:2394-2436): midpoint pivot, Hoare-style
+// two-pointer partition, recursion on both parts. Kept step for step so
+// that ties at the truncation cut are ordered the same way. 1-based.
+void quicksort_desc(std::vector<double> &a, std::vector<int> &idx, int lower, int upper) {
+  int i = lower, j = upper;
+  const int mid = (int)((lower + upper) / 2.0);
+  const double piv = a[mid];
+  while (i <= j) {
+    while (a[i] > piv) ++i;
+    while (a[j] < piv) --j;
+    if (i <= j) {
+      std::swap(a[i], a[j]);
+      std::swap(idx[i], idx[j]);
+      ++i;
+      --j;
+    }
+  }
+  if (lower < j) quicksort_desc(a, idx, lower, j);
+  if (i < upper) quicksort_desc(a, idx, i, upper);
+}
+
+// Kept-state selection of Get_truncation_operator (den_mat.f90:255-364):
+// eig (ascending per sector) -> kept counts per sector and scalars.
+void select_states(int32_t nsec, const int32_t *dim, const int32_t *twos, const double *eig, int32_t kept_min,
+                   int32_t kept_max, double trun_err, int32_t *kept, double *scalars) {
+  int64_t tot = 0;
+  for (int32_t s = 0; s < nsec; ++s) tot += dim[s];
+  const int n = (int)tot;
+  std::vector<double> ev(n + 1);
+  std::vector<int> idx(n + 1), owner(n + 1);
+  std::vector<int> wts(n + 1);
+  int pos = 0;
+  for (int32_t s = 0; s < nsec; ++s)
+    for (int x = 0; x < dim[s]; ++x) {
+      ++pos;
+      ev[pos] = eig[pos - 1];
+      idx[pos] = pos;
+      owner[pos] = s;
+      wts[pos] = twos[s] + 1;
+    }
+  // U(1)-resolved entropy on the raw spectrum (den_mat.f90:275-282)
+  double entropy1 = 0.0;
+  for (int i = 1; i <= n; ++i)
+    if (ev[i] >= 1.0e-15) {
+      const double v = ev[i] / wts[i];
+      entropy1 -= v * std::log(v) * wts[i];
+    }
+  if (n > 0) quicksort_desc(ev, idx, 1, n);
+  double tot_pm = 0.0;
+  for (int i = 1; i <= n; ++i) tot_pm += ev[i];
+  for (int i = 1; i <= n; ++i) ev[i] /= tot_pm;
+  int num_state;
+  double err = 0.0;
+  if (n <= kept_min) {
+    num_state = n;
+    err = 0.0;
+  } else {
+    for (int i = kept_min + 1; i <= n; ++i) err += ev[i];
+    num_state = kept_min;
+    while (err > trun_err) {
+      ++num_state;
+      err -= ev[num_state];
+      if (err <= trun_err || num_state >= kept_max) break;
+    }
+  }
+  double ent = 0.0;
+  for (int i = 1; i <= n; ++i)
+    if (ev[i] >= 1.0e-15) ent -= ev[i] * std::log(ev[i]);
+  for (int32_t s = 0; s < nsec; ++s) kept[s] = 0;
+  for (int k = 1; k <= num_state; ++k) kept[owner[idx[k]]] += 1;
+  scalars[0] = num_state;
+  scalars[1] = err;
+  scalars[2] = ent;
+  scalars[3] = entropy1;
+}
+
+int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_t *twos, double2 *d_rho,
+                    const std::vector<int64_t> &rho_off, int32_t kept_min, int32_t kept_max, double trun_err,
+                    double *eigval_out, int32_t *kept_out, double *u_out, double *scalars_out) {
+  std::vector<int64_t> w_off(nsec);
+  int64_t wt = 0;
+  for (int32_t s = 0; s < nsec; ++s) {
+    w_off[s] = wt;
+    wt += dim[s];
+  }
+  DevBuf dw;
+  TSC_HIP_CHECK(hipMalloc(&dw.p, sizeof(double) * std::max<int64_t>(wt, 1)));
+  int rc = eig_sectors(ctx, nsec, dim, d_rho, rho_off, (double *)dw.p, w_off);
+  if (rc) return rc;
+  std::vector<double> w(std::max<int64_t>(wt, 1));
+  TSC_HIP_CHECK(hipMemcpy(w.data(), dw.p, sizeof(double) * wt, hipMemcpyDeviceToHost));
+  select_states(nsec, dim, twos, w.data(), kept_min, kept_max, trun_err, kept_out, scalars_out);
+  if (eigval_out) std::memcpy(eigval_out, w.data(), sizeof(double) * wt);
+  // U_s = last kept_s eigenvector columns of sector s (den_mat.f90:392-398)
+  int64_t uoff = 0;
+  for (int32_t s = 0; s < nsec; ++s) {
+    const int d = dim[s], k = kept_out[s];
+    if (k == 0) continue;
+    const double2 *src = d_rho + rho_off[s] + (int64_t)d * (d - k);
+    TSC_HIP_CHECK(hipMemcpyAsync(u_out + 2 * uoff, src, sizeof(double2) * (int64_t)d * k, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    uoff += (int64_t)d * k;
+  }
+  TSC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return TSC_OK;
+}
+
+}  // namespace
+
+extern "C" int tsc_density_matrix(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, const int32_t *sys_dim,
+                                  const int32_t *env_dim, int side, double *rho_out) {
+  TSC_REQUIRE(ctx && psi && (nsec == 0 || (sys_dim && env_dim)) && rho_out, "tsc_density_matrix arguments");
+  TSC_REQUIRE(side == 0 || side == 1, "side must be 0 (sys) or 1 (env)");
+  int64_t rho_elems = 0;
+  for (int32_t s = 0; s < nsec; ++s) {
+    const int64_t d = side == 0 ? sys_dim[s] : env_dim[s];
+    rho_elems += d * d;
+  }
+  DevBuf rho;
+  TSC_HIP_CHECK(hipMalloc(&rho.p, sizeof(double2) * std::max<int64_t>(rho_elems, 1)));
+  std::vector<int64_t> off;
+  int rc = build_rho(ctx, psi->d, psi->n, nsec, sys_dim, env_dim, side, (double2 *)rho.p, rho_elems, off);
+  if (rc) return rc;
+  TSC_HIP_CHECK(hipMemcpy(rho_out, rho.p, sizeof(double2) * rho_elems, hipMemcpyDeviceToHost));
+  return TSC_OK;
+}
+
+extern "C" int tsc_truncation_operator(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_t *twos,
+                                       const double *rho, int32_t kept_min, int32_t kept_max, double trun_err,
+                                       double *eigval_out, int32_t *kept_out, double *u_out, double *scalars_out) {
+  TSC_REQUIRE(ctx && (nsec == 0 || (dim && twos && rho)) && kept_out && u_out && scalars_out,
+              "tsc_truncation_operator arguments");
+  int64_t rho_elems = 0;
+  std::vector<int64_t> off(nsec);
+  for (int32_t s = 0; s < nsec; ++s) {
+    TSC_REQUIRE(dim[s] >= 0 && twos[s] >= 0, "negative dim / 2S");
+    off[s] = rho_elems;
+    rho_elems += (int64_t)dim[s] * dim[s];
+  }
+  DevBuf drho;
+  TSC_HIP_CHECK(hipMalloc(&drho.p, sizeof(double2) * std::max<int64_t>(rho_elems, 1)));
+  TSC_HIP_CHECK(hipMemcpy(drho.p, rho, sizeof(double2) * rho_elems, hipMemcpyHostToDevice));
+  return truncate_device(ctx, nsec, dim, twos, (double2 *)drho.p, off, kept_min, kept_max, trun_err, eigval_out,
+                         kept_out, u_out, scalars_out);
+}
+
+extern "C" int tsc_rho_truncate(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, const int32_t *sys_dim,
+                                const int32_t *env_dim, const int32_t *twos, int side, int32_t kept_min,
+                                int32_t kept_max, double trun_err, double *eigval_out, int32_t *kept_out,
+                                double *u_out, double *scalars_out) {
+  TSC_REQUIRE(ctx && psi && (nsec == 0 || (sys_dim && env_dim && twos)) && kept_out && u_out && scalars_out,
+              "tsc_rho_truncate arguments");
+  TSC_REQUIRE(side == 0 || side == 1, "side must be 0 (sys) or 1 (env)");
+  std::vector<int32_t> dim(nsec);
+  int64_t rho_elems = 0;
+  for (int32_t s = 0; s < nsec; ++s) {
+    dim[s] = side == 0 ? sys_dim[s] : env_dim[s];
+    rho_elems += (int64_t)dim[s] * dim[s];
+  }
+  DevBuf rho;
+  TSC_HIP_CHECK(hipMalloc(&rho.p, sizeof(double2) * std::max<int64_t>(rho_elems, 1)));
+  std::vector<int64_t> off;
+  int rc = build_rho(ctx, psi->d, psi->n, nsec, sys_dim, env_dim, side, (double2 *)rho.p, rho_elems, off);
+  if (rc) return rc;
+  return truncate_device(ctx, nsec, dim.data(), twos, (double2 *)rho.p, off, kept_min, kept_max, trun_err,
+                         eigval_out, kept_out, u_out, scalars_out);
+}
+
+extern "C" int tsc_update_trun(tsc_ctx *ctx, int32_t nblk, const int32_t *row, const int32_t *col, const int32_t *kl,
+                               const int32_t *kr, const int64_t *ori_off, const int64_t *ul_off,
+                               const int64_t *ur_off, const int64_t *eff_off, const double *ori, int64_t ori_elems,
+                               const double *u, int64_t u_elems, double *eff, int64_t eff_elems, int prec) {
+  TSC_REQUIRE(ctx && nblk >= 0 && eff, "tsc_update_trun arguments");
+  TSC_REQUIRE(nblk == 0 || (row && col && kl && kr && ori_off && ul_off && ur_off && eff_off && ori && u),
+              "tsc_update_trun arrays");
+  std::vector<tsc_contrib> c1(nblk), c2(nblk);
+  std::vector<tsc_outblock> o1(nblk), o2(nblk);
+  int64_t moff = 0;
+  for (int32_t i = 0; i < nblk; ++i) {
+    TSC_REQUIRE(row[i] >= 0 && col[i] >= 0 && kl[i] >= 0 && kr[i] >= 0, "negative block dims");
+    // stage 1: mid = O (row x col) . U_rhs (col x kr)
+    tsc_contrib &a = c1[i];
+    std::memset(&a, 0, sizeof(a));
+    a.a_off = ori_off[i];
+    a.lda = std::max(1, row[i]);
+    a.a_buf = TSC_BUF_OPS;
+    a.a_op = TSC_OP_N;
+    a.b_off = ur_off[i];
+    a.ldb = std::max(1, col[i]);
+    a.b_buf = TSC_BUF_IN;
+    a.b_op = TSC_OP_N;
+    a.k = col[i];
+    a.coef_re = 1.0;
+    tsc_outblock &ob1 = o1[i];
+    ob1.c_off = moff;
+    ob1.ldc = std::max(1, row[i]);
+    ob1.m = row[i];
+    ob1.n = kr[i];
+    ob1.c_buf = TSC_BUF_MID;
+    ob1.cbeg = i;
+    ob1.cend = i + 1;
+    ob1.mode = 0;
+    // stage 2: eff = U_lhs^H (kl x row) . mid (row x kr)
+    tsc_contrib &b = c2[i];
+    std::memset(&b, 0, sizeof(b));
+    b.a_off = ul_off[i];
+    b.lda = std::max(1, row[i]);
+    b.a_buf = TSC_BUF_IN;
+    b.a_op = TSC_OP_C;
+    b.b_off = moff;
+    b.ldb = std::max(1, row[i]);
+    b.b_buf = TSC_BUF_MID;
+    b.b_op = TSC_OP_N;
+    b.k = row[i];
+    b.coef_re = 1.0;
+    tsc_outblock &ob2 = o2[i];
+    ob2.c_off = eff_off[i];
+    ob2.ldc = std::max(1, kl[i]);
+    ob2.m = kl[i];
+    ob2.n = kr[i];
+    ob2.c_buf = TSC_BUF_OUT;
+    ob2.cbeg = i;
+    ob2.cend = i + 1;
+    ob2.mode = 0;
+    moff += (int64_t)row[i] * kr[i];
+  }
+  tsc_plan *p = nullptr;
+  int rc = plan_create_impl(ctx, ori, nullptr, ori_elems, moff, c1.data(), nblk, o1.data(), nblk, c2.data(), nblk,
+                            o2.data(), nblk, u_elems, eff_elems, &p);
+  if (rc) return rc;
+  rc = tsc_hpsi_host(p, u, eff, prec);
+  tsc_plan_destroy(p);
+  return rc;
+}
+
+// Racah formula for the 6j symbol, doubled arguments (the role of W6JS,
+// block.f90:3744-3795): {j1 j2 j3; l1 l2 l3} with triads (j1 j2 j3),
+// (j1 l2 l3), (l1 j2 l3), (l1 l2 j3). Factorials exact in double to 170!.
+extern "C" double tsc_w6j(int32_t j1, int32_t j2, int32_t j3, int32_t l1, int32_t l2, int32_t l3) {
+  static double fact[171];
+  static bool init = false;
+  if (!init) {
+    fact[0] = 1.0;
+    for (int i = 1; i <= 170; ++i) fact[i] = fact[i - 1] * (double)i;
+    init = true;
+  }
+  auto tri_ok = [](int a, int b, int c) {
+    return a >= 0 && b >= 0 && c >= 0 && c <= a + b && c >= std::abs(a - b) && ((a + b + c) % 2 == 0);
+  };
+  if (!tri_ok(j1, j2, j3) || !tri_ok(j1, l2, l3) || !tri_ok(l1, j2, l3) || !tri_ok(l1, l2, j3)) return 0.0;
+  auto delta = [&](int a, int b, int c) {
+    return std::sqrt(fact[(a + b - c) / 2] * fact[(a - b + c) / 2] * fact[(-a + b + c) / 2] / fact[(a + b + c) / 2 + 1]);
+  };
+  const int a1 = (j1 + j2 + j3) / 2, a2 = (j1 + l2 + l3) / 2, a3 = (l1 + j2 + l3) / 2, a4 = (l1 + l2 + j3) / 2;
+  const int b1 = (j1 + j2 + l1 + l2) / 2, b2 = (j2 + j3 + l2 + l3) / 2, b3 = (j3 + j1 + l3 + l1) / 2;
+  const int tmin = std::max(std::max(a1, a2), std::max(a3, a4));
+  const int tmax = std::min(b1, std::min(b2, b3));
+  if (tmax + 1 > 170) return std::nan("");
+  double sum = 0.0;
+  for (int t = tmin; t <= tmax; ++t) {
+    const double den = fact[t - a1] * fact[t - a2] * fact[t - a3] * fact[t - a4] * fact[b1 - t] * fact[b2 - t] *
+                       fact[b3 - t];
+    const double term = fact[t + 1] / den;
+    sum += (t % 2 == 0) ? term : -term;
+  }
+  return sum * delta(j1, j2, j3) * delta(j1, l2, l3) * delta(l1, j2, l3) * delta(l1, l2, j3);
+}
