@@ -19,7 +19,7 @@ def test_ed_closed_forms():
 
 
 def test_lattice_is_triangular():
-    Lx, Ny = 4, 4
+    Lx, Ny = 5, 4
     b = tj_model.bonds(Lx, Ny, 1.0, 0.5, 1.0, 0.25)
     pos = {}
     for ix in range(1, Lx + 1):
@@ -41,7 +41,7 @@ def test_lattice_is_triangular():
     for a, c in nnn:
         assert abs(dist(a, c) - np.sqrt(3)) < 1e-12
     # bulk coordination: 6 NN and 6 NNN around an interior site
-    inner = 2 + 1 * Ny + 1
+    inner = 2 + 2 * Ny
     assert sum(1 for a, c in nn if inner in (a, c)) == 6
     assert sum(1 for a, c in nnn if inner in (a, c)) == 6
     # open x: 3*Ny*Lx - 2*Ny ... count every NN pair once

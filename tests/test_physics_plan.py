@@ -1,0 +1,82 @@
+"""The product's sub-block task plans (model_wave, renormalisation)
+evaluated on the host against the oracle's dense SU(2) superblock, on
+identical blocks. CPU only: the device kernels are checked elsewhere."""
+import numpy as np
+import pytest
+
+from oracle import dmrg_exact, su2_dense, tj_model
+from tests.model_convert import block_to_product, oracle_superblock
+from tests.plan_eval import apply_numpy
+from tsc_tri_t1t2j1j2_amd import hamiltonian as ham
+
+CASES = [(2, 3, 4, 0, 2), (2, 3, 5, 1, 2), (2, 4, 6, 0, 3), (2, 4, 6, 0, 2), (3, 3, 8, 0, 3), (2, 4, 7, 1, 1),
+         (3, 4, 10, 0, 4)]
+
+
+def _setup(Lx, Ny, Ne, S2, l, t2=0.2, J2=0.016):
+    N = Lx * Ny
+    b = tj_model.bonds(Lx, Ny, 1.0, t2, 0.4, J2)
+    sys_blk, env_blk, _ = dmrg_exact.build_chain_blocks(N, b, l)
+    osb = oracle_superblock(sys_blk, env_blk, l + 1, l + 2, b, Ne, S2)
+    psb = ham.Superblock(block_to_product(sys_blk), block_to_product(env_blk), l + 1, l + 2, b, Ne, S2)
+    return osb, psb
+
+
+@pytest.mark.parametrize("Lx,Ny,Ne,S2,l", CASES)
+def test_model_wave_plan_matches_oracle(Lx, Ny, Ne, S2, l):
+    osb, psb = _setup(Lx, Ny, Ne, S2, l)
+    assert psb.super.dim == osb.n
+    assert [(x, y) for (x, y, _, _) in psb.super.sub] == [(x, y) for (x, y, _, _) in osb.sb]
+    rng = np.random.default_rng(Lx * 100 + Ny * 10 + l)
+    v = rng.normal(size=osb.n) + 1j * rng.normal(size=osb.n)
+    pb = psb.plan_builder()
+    got = apply_numpy(pb, v, osb.n)
+    want = osb.matvec(v)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_renormalize_plan_matches_oracle():
+    Lx, Ny = 3, 4
+    N = Lx * Ny
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    sys_blk, _, nb = dmrg_exact.build_chain_blocks(N, b, 5)
+    new_site = 6
+    to_new = [(i, t, J) for (i, t, J) in nb[new_site] if i in sys_blk.sites]
+    keep = [i for i in sys_blk.sites + [new_site] if any(j > new_site for (j, _, _) in nb.get(i, []))]
+    eb = su2_dense.enlarge(sys_blk, new_site, to_new, keep)
+    rng = np.random.default_rng(3)
+    U, order = {}, []
+    for x, (Nx, Jx, dx) in enumerate(eb.sectors):
+        if x % 3 == 1:
+            continue     # drop a few sectors
+        k = max(1, dx // 2)
+        q = np.linalg.qr(rng.normal(size=(dx, k)) + 1j * rng.normal(size=(dx, k)))[0]
+        U[x] = q
+        order.append(x)
+    order = order[::-1]  # a non-trivial new sector order
+    want = su2_dense.truncate(eb, [U.get(x) for x in range(len(eb.sectors))])
+    rp = ham.renormalize_plan(block_to_product(sys_blk), new_site, to_new, keep, U, order)
+    res = apply_numpy(rp.builder, rp.uin, rp.out_len)
+    got = ham.renormalize_result(rp, res)
+    # oracle sectors are in enlarged order; map ours back
+    pos = {x: n for n, x in enumerate(order)}
+    okeep = [x for x in range(len(eb.sectors)) if x in U]
+    oidx = {x: n for n, x in enumerate(okeep)}
+    assert sorted(got.sectors) == sorted(want.sectors)
+
+    def check(gop, wop):
+        for (b2, k2), m in wop.blocks.items():
+            gb = gop.blocks.get((pos[okeep[b2]], pos[okeep[k2]]))
+            if gb is None:
+                assert np.abs(m).max() < 1e-13
+            else:
+                assert np.allclose(gb, m, atol=1e-12)
+        for (b2, k2), m in gop.blocks.items():
+            key = (oidx[order[b2]], oidx[order[k2]])
+            if key not in wop.blocks:
+                assert np.abs(m).max() < 1e-13
+
+    check(got.ham, want.H)
+    for i in keep:
+        for name in ("c", "S", "n"):
+            check(got.ops[i][name], want.ops[i][name])

@@ -1,0 +1,57 @@
+"""YC triangular cylinder and t1-t2-J1-J2 couplings (the neighbour tables
+of hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/dmrg_sub.f90:334-644, triangle=1,
+open x, periodic y; parameters of DMRG_code/main.f90:54-99).
+
+Sites i = iy + (ix-1)*Ny (1-based). Nearest-neighbour displacements (dx,dy)
+= (0,1), (1,0), (1,-1); next-nearest (1,1), (2,-1), (1,-2), with (2,-1)
+omitted on Ny = 3 cylinders as in the reference. One coupling per site
+pair. The DMRG chain order is the site order.
+"""
+NN = ((0, 1), (1, 0), (1, -1))
+NNN = ((1, 1), (2, -1), (1, -2))
+
+
+def site(ix, iy, Ny):
+    return iy + (ix - 1) * Ny
+
+
+def bonds(Lx, Ny, t1, t2, J1, J2, ncols=None, present=None):
+    """[(i, j, t, J)] with i < j. ncols: columns of the cylinder the table
+    is built on (warm-up superblocks use a shorter one); present: subset of
+    sites kept."""
+    nc = Lx if ncols is None else ncols
+    pairs = {}
+    for shell, (t, J) in ((NN, (t1, J1)), (NNN, (t2, J2))):
+        if t == 0 and J == 0:
+            continue
+        for (dx, dy) in shell:
+            if Ny == 3 and (dx, dy) == (2, -1):
+                continue
+            if Ny == 2 and (dx, dy) == (0, 1):
+                # a 2-leg cylinder keeps a single y bond per rung
+                pass
+            for ix in range(1, nc + 1):
+                for iy in range(1, Ny + 1):
+                    jx = ix + dx
+                    if jx < 1 or jx > nc:
+                        continue
+                    jy = (iy - 1 + dy) % Ny + 1
+                    a, b = site(ix, iy, Ny), site(jx, jy, Ny)
+                    if a == b:
+                        continue
+                    key = (min(a, b), max(a, b))
+                    if key not in pairs:
+                        pairs[key] = (t, J)
+    out = []
+    for (a, b), (t, J) in sorted(pairs.items()):
+        if present is not None and (a not in present or b not in present):
+            continue
+        out.append((a, b, t, J))
+    return out
+
+
+def target_electrons(N, delta=12):
+    """qn0(N,1) of main.f90:93-97: electrons for a chain of N sites at hole
+    doping 1/delta, rounded to an even number."""
+    q = N // 2 - (N + 2 * delta - 1) // (2 * delta)
+    return 2 * q
