@@ -99,6 +99,12 @@ int tsc_version(void);
 int tsc_ctx_create(int device, tsc_ctx **out);
 int tsc_ctx_destroy(tsc_ctx *ctx);
 int tsc_ctx_sync(tsc_ctx *ctx);
+/*
+ * Accumulator row layout of the f64 / f32 / bf16 16x16 MFMAs found by the
+ * probe run at context creation (0: row = 4*(lane/16)+r, 1: row =
+ * lane/16+4r). Diagnostic; has no counterpart in the reference.
+ */
+int tsc_ctx_mfma_layout(const tsc_ctx *ctx, int32_t *modes3);
 
 /* ---- resident vectors (psi, Lanczos vectors) ---------------------------- */
 int tsc_vec_create(tsc_ctx *ctx, int64_t n, tsc_vec **out);

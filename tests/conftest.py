@@ -24,5 +24,5 @@ def _name_the_case(request):
 def ctx():
     from tsc_tri_t1t2j1j2_amd import capi
     c = capi.Context(0)
+    print(f"[gpu] MFMA accumulator layouts (f64, f32, bf16): {c.mfma_layout()}", flush=True)
     yield c
-    c.close()

@@ -1,0 +1,111 @@
+"""Device parity of the SU(2) physics path: model_wave and block
+renormalisation plans against the oracle's dense superblock, a truncating
+DMRG step against the oracle's step, and whole DMRG runs against exact
+diagonalisation."""
+import numpy as np
+import pytest
+
+from oracle import dmrg_exact, su2_dense, tj_ed, tj_model
+from oracle import lanczos as olanczos
+from oracle import truncation as otrunc
+from tests.model_convert import block_to_product, oracle_superblock
+from tsc_tri_t1t2j1j2_amd import capi
+from tsc_tri_t1t2j1j2_amd import hamiltonian as ham
+from tsc_tri_t1t2j1j2_amd.dmrg import DMRG, start_vector
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(Lx, Ny, Ne, S2, l):
+    N = Lx * Ny
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    sys_blk, env_blk, nb = dmrg_exact.build_chain_blocks(N, b, l)
+    osb = oracle_superblock(sys_blk, env_blk, l + 1, l + 2, b, Ne, S2)
+    psb = ham.Superblock(block_to_product(sys_blk), block_to_product(env_blk), l + 1, l + 2, b, Ne, S2)
+    return osb, psb, sys_blk, env_blk, b, nb
+
+
+@pytest.mark.parametrize("prec,tol", [(capi.PREC_F64, 1e-12), (capi.PREC_F32, 1e-5), (capi.PREC_BF16X3, 2e-4)])
+@pytest.mark.parametrize("case", [(2, 4, 6, 0, 3), (3, 4, 10, 0, 4), (2, 4, 7, 1, 1)])
+def test_model_wave_device_matches_oracle(ctx, case, prec, tol):
+    osb, psb, *_ = _setup(*case)
+    plan = psb.plan_builder().build(ctx, psb.super.dim, psb.super.dim)
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=osb.n) + 1j * rng.normal(size=osb.n)
+    got = plan.apply_host(v, prec)
+    want = osb.matvec(v)
+    assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want)
+
+
+def test_lanczos_on_superblock_equals_ed(ctx):
+    Lx, Ny, Ne, S2, l = 3, 3, 8, 0, 3
+    osb, psb, *_ = _setup(Lx, Ny, Ne, S2, l)
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b, twoS=S2)
+    plan = psb.plan_builder().build(ctx, psb.super.dim, psb.super.dim)
+    psi = ctx.vec_from(start_vector(psb.super, 7))
+    e, _ = plan.lanczos(psi, tol=1e-12)
+    assert abs(e - e_ed) < 1e-9
+    psi32 = ctx.vec_from(start_vector(psb.super, 7))
+    e32, _ = plan.lanczos(psi32, tol=1e-12, prec=capi.PREC_BF16X3, refine_iters=40)
+    assert abs(e32 - e_ed) < 1e-9
+
+
+def test_truncating_step_matches_oracle(ctx):
+    """Same blocks and start vector: energy, kept split and the spectrum of
+    the renormalised block Hamiltonian agree with the oracle's step."""
+    Lx, Ny, Ne, l = 3, 4, 10, 4
+    osb, psb, sys_blk, env_blk, b, nb = _setup(Lx, Ny, Ne, 0, l)
+    psi0 = start_vector(psb.super, 11)
+    plan = psb.plan_builder().build(ctx, psb.super.dim, psb.super.dim)
+    v = ctx.vec_from(psi0)
+    e, _ = plan.lanczos(v, tol=1e-12)
+    e_or, psi_or, _ = olanczos.lanczos(osb.matvec, psi0, tol=1e-12)
+    assert abs(e - e_or) < 1e-10
+    kmin, kmax, terr = 12, 30, 1e-7
+    sd = [dx for (_, _, dx, _) in psb.super.sub]
+    ed = [dy for (_, _, _, dy) in psb.super.sub]
+    xs = [x for (x, _, _, _) in psb.super.sub]
+    twos = [psb.sys_bs.sectors[x][1] for x in xs]
+    got = capi.rho_truncate(ctx, v, sd, ed, twos, 0, kmin, kmax, terr)
+    secs = [osb.sector(psi_or, x, y) for (x, y, _, _) in osb.sb]
+    want = otrunc.truncation_operator(otrunc.density_matrix(secs, 0), twos, kmin, kmax, terr)
+    assert list(got["kept"]) == list(want["kept"])
+    assert abs(got["trun_err"] - want["trun_err"]) < 1e-10
+    # renormalise the sys block with both U's; compare gauge-invariant spectra
+    s1 = l + 1
+    to_new = [(i, t, J) for (i, t, J) in nb[s1] if i in sys_blk.sites]
+    keep = [i for i in sys_blk.sites + [s1] if any(j > s1 for (j, _, _) in nb.get(i, []))]
+    U, order = {}, []
+    for x, u, k in zip(xs, got["U"], got["kept"]):
+        if k:
+            U[x] = u
+            order.append(x)
+    newb = ham.renormalize(ctx, block_to_product(sys_blk), s1, to_new, keep, U, order)
+    eb = su2_dense.enlarge(sys_blk, s1, to_new, keep)
+    Uo = [None] * len(eb.sectors)
+    for x, u in zip(xs, want["U"]):
+        Uo[x] = u
+    ob = su2_dense.truncate(eb, Uo)
+    ospec = sorted(np.round(np.linalg.eigvalsh(m), 9).tolist() for m in ob.H.blocks.values())
+    gspec = sorted(np.round(np.linalg.eigvalsh(m), 9).tolist() for (bk, kk), m in newb.ham.blocks.items() if bk == kk)
+    assert ospec == gspec
+
+
+@pytest.mark.parametrize("Lx,Ny,Ne", [(3, 3, 8), (2, 4, 6)])
+def test_dmrg_untruncated_equals_ed(ctx, Lx, Ny, Ne):
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
+    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-10)
+    e = d.run()
+    assert abs(e - e_ed) < 1e-8, (e, e_ed)
+
+
+def test_dmrg_truncated_is_variational_and_close(ctx):
+    Lx, Ny, Ne = 3, 4, 10
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
+    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=20, kept_max=60, sweeps=3, lanzero=1e-9, trun_err=1e-10)
+    e = d.run()
+    assert e >= e_ed - 1e-9
+    assert (e - e_ed) / (Lx * Ny) < 1e-4

@@ -34,7 +34,7 @@ OUTBLOCK_DTYPE = np.dtype({
 
 # every symbol declared in include/tsc_hotpath.h
 EXPORTS = [
-    "tsc_last_error", "tsc_version", "tsc_ctx_create", "tsc_ctx_destroy", "tsc_ctx_sync",
+    "tsc_last_error", "tsc_version", "tsc_ctx_create", "tsc_ctx_destroy", "tsc_ctx_sync", "tsc_ctx_mfma_layout",
     "tsc_vec_create", "tsc_vec_destroy", "tsc_vec_upload", "tsc_vec_download", "tsc_vec_dot_re",
     "tsc_vec_norm", "tsc_vec_dptr", "tsc_vec_fill_gaussian", "tsc_vec_axpby", "tsc_vec_copy",
     "tsc_plan_create", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
@@ -74,6 +74,7 @@ def lib():
         "tsc_ctx_create": (c_int, [c_int, P(c_void_p)]),
         "tsc_ctx_destroy": (c_int, [c_void_p]),
         "tsc_ctx_sync": (c_int, [c_void_p]),
+        "tsc_ctx_mfma_layout": (c_int, [c_void_p, P(c_int32)]),
         "tsc_vec_create": (c_int, [c_void_p, c_int64, P(c_void_p)]),
         "tsc_vec_destroy": (c_int, [c_void_p]),
         "tsc_vec_upload": (c_int, [c_void_p, c_void_p]),
@@ -137,6 +138,8 @@ class Context:
         self.h = h
 
     def close(self):
+        """Destroy the context. Vectors and plans made from it must not be
+        used afterwards; their own destructors then skip the device free."""
         if self.h:
             lib().tsc_ctx_destroy(self.h)
             self.h = None
@@ -146,6 +149,11 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def mfma_layout(self):
+        m = (c_int32 * 3)()
+        check(lib().tsc_ctx_mfma_layout(self.h, m))
+        return list(m)
 
     def sync(self):
         check(lib().tsc_ctx_sync(self.h))
@@ -171,9 +179,9 @@ class Vec:
         self.h = h
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().tsc_vec_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -231,9 +239,9 @@ class Plan:
         self.ops_elems = int(ops_elems)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().tsc_plan_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

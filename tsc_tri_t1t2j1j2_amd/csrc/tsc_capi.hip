@@ -127,6 +127,12 @@ using namespace tsc;
 extern "C" const char *tsc_last_error(void) { return g_err.c_str(); }
 extern "C" int tsc_version(void) { return 1; }
 
+extern "C" int tsc_ctx_mfma_layout(const tsc_ctx *c, int32_t *modes) {
+  TSC_REQUIRE(c && modes, "tsc_ctx_mfma_layout arguments");
+  for (int i = 0; i < 3; ++i) modes[i] = c->mfma_rowmode[i];
+  return TSC_OK;
+}
+
 extern "C" int tsc_ctx_create(int device, tsc_ctx **out) {
   TSC_REQUIRE(out, "tsc_ctx_create: null out");
   int ndev = 0;
@@ -143,6 +149,13 @@ extern "C" int tsc_ctx_create(int device, tsc_ctx **out) {
   TSC_HIP_CHECK(hipMalloc((void **)&c->d_coef, sizeof(double) * kMaxVecs));
   TSC_HIP_CHECK(hipHostMalloc((void **)&c->h_ptrs, sizeof(void *) * kMaxVecs, hipHostMallocDefault));
   TSC_HIP_CHECK(hipHostMalloc((void **)&c->h_coef, sizeof(double) * kMaxVecs, hipHostMallocDefault));
+  {
+    int rc = probe_layouts(c->stream, c->mfma_rowmode);
+    if (rc) {
+      tsc_ctx_destroy(c);
+      return rc;
+    }
+  }
   *out = c;
   return TSC_OK;
 }
@@ -422,9 +435,9 @@ int enqueue_hpsi(tsc_plan *p, const double2 *in, double2 *out, int prec, bool ev
   b.p[TSC_BUF_MID] = (prec == TSC_PREC_F64) ? (const void *)p->d_mid64 : (const void *)p->d_mid32;
   b.p[TSC_BUF_OUT] = out;
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[0], s));
-  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[0], p->d_c[0], p->d_t[0], p->nt[0], s));
+  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[0], p->d_c[0], p->d_t[0], p->nt[0], s, p->ctx->mfma_rowmode));
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[1], s));
-  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[1], p->d_c[1], p->d_t[1], p->nt[1], s));
+  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[1], p->d_c[1], p->d_t[1], p->nt[1], s, p->ctx->mfma_rowmode));
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[2], s));
   return TSC_OK;
 }

@@ -50,6 +50,13 @@ __device__ __forceinline__ void ld_op(const Bufs &B, int buf, int64_t off, int l
   if (op == TSC_OP_C || op == TSC_OP_CONJ) im = -im;
 }
 
+// Row of accumulator element r held by `lane` in a 16x16 MFMA output tile
+// (the column is lane & 15). The two candidate layouts are told apart once
+// per context by probe_layouts(); rowmode selects the one the device uses.
+__device__ __forceinline__ int acc_row(int rowmode, int lane, int r) {
+  return rowmode ? ((lane >> 4) + 4 * r) : (4 * (lane >> 4) + r);
+}
+
 template <typename CT>
 __device__ __forceinline__ void cmul(CT ar, CT ai, CT br, CT bi, CT &cr, CT &ci) {
   cr = ar * br - ai * bi;
@@ -78,7 +85,7 @@ struct MfmaTraits<float> {
 template <typename CT, int TK>
 __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outblock *__restrict__ blocks,
                                                       const tsc_contrib *__restrict__ contribs,
-                                                      const Tile *__restrict__ tiles) {
+                                                      const Tile *__restrict__ tiles, int rowmode) {
   typedef typename MfmaTraits<CT>::acc_t acc_t;
   constexpr int PAD = (sizeof(CT) == 8) ? 16 : 32;  // row padding (elements)
   __shared__ CT As_r[TK][TM + PAD], As_i[TK][TM + PAD];
@@ -120,7 +127,7 @@ __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outbl
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+            const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
             const int j = j0 + wn + b * 16 + (lane & 15);
             if (i < m && j < n) {
               CT xr, xi, pr, pi;
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outbl
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+        const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
         const int j = j0 + wn + b * 16 + (lane & 15);
         if (i < m && j < n) {
           const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
@@ -246,7 +253,7 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
 
 __global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const tsc_outblock *__restrict__ blocks,
                                                              const tsc_contrib *__restrict__ contribs,
-                                                             const Tile *__restrict__ tiles) {
+                                                             const Tile *__restrict__ tiles, int rowmode) {
   constexpr int TK = 32;
   // layout [row][k] so that each lane reads 8 consecutive k (16 bytes)
   constexpr int KP = TK + 8;
@@ -288,7 +295,7 @@ __global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const ts
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+            const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
             const int j = j0 + wn + b * 16 + (lane & 15);
             if (i < m && j < n) {
               float xr, xi, pr, pi;
@@ -393,7 +400,7 @@ __global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const ts
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = i0 + wm + a * 16 + 4 * (lane >> 4) + r;
+        const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
         const int j = j0 + wn + b * 16 + (lane & 15);
         if (i < m && j < n) {
           const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
@@ -420,19 +427,97 @@ __global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const ts
       }
 }
 
+// ---------------------------------------------------------------------
+// Layout probe: C = A B with A[i][k] = i (or 1) and B[k][j] = 1 (or j)
+// for every k; out[kind][pass][lane*4 + r] = C element held in register r,
+// i.e. K*row or K*col of that element. Uses only the row-of-A = lane & 15
+// and col-of-B = lane & 15 conventions of the 16x16 MFMA family.
+__global__ void probe_kernel(float *out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 15;
+  // f64 16x16x4
+  {
+    d4 c1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)i, 1.0, d4{0, 0, 0, 0}, 0, 0, 0);
+    d4 c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(1.0, (double)i, d4{0, 0, 0, 0}, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) {
+      out[0 * 512 + lane * 4 + r] = (float)c1[r];
+      out[0 * 512 + 256 + lane * 4 + r] = (float)c2[r];
+    }
+  }
+  // f32 16x16x4
+  {
+    f4 c1 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)i, 1.0f, f4{0, 0, 0, 0}, 0, 0, 0);
+    f4 c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(1.0f, (float)i, f4{0, 0, 0, 0}, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) {
+      out[1 * 512 + lane * 4 + r] = c1[r];
+      out[1 * 512 + 256 + lane * 4 + r] = c2[r];
+    }
+  }
+  // bf16 16x16x32
+  {
+    bf8 vi, v1;
+    for (int e = 0; e < 8; ++e) {
+      vi[e] = (__bf16)(float)i;
+      v1[e] = (__bf16)1.0f;
+    }
+    f4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vi, v1, f4{0, 0, 0, 0}, 0, 0, 0);
+    f4 c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v1, vi, f4{0, 0, 0, 0}, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) {
+      out[2 * 512 + lane * 4 + r] = c1[r];
+      out[2 * 512 + 256 + lane * 4 + r] = c2[r];
+    }
+  }
+}
+
+int probe_layouts(hipStream_t stream, int modes[3]) {
+  float *d = nullptr;
+  TSC_HIP_CHECK(hipMalloc((void **)&d, sizeof(float) * 3 * 512));
+  hipLaunchKernelGGL(probe_kernel, dim3(1), dim3(64), 0, stream, d);
+  hipError_t le = hipGetLastError();
+  std::vector<float> h(3 * 512);
+  hipError_t ce = hipMemcpyAsync(h.data(), d, sizeof(float) * 3 * 512, hipMemcpyDeviceToHost, stream);
+  hipError_t se = hipStreamSynchronize(stream);
+  hipFree(d);
+  TSC_HIP_CHECK(le);
+  TSC_HIP_CHECK(ce);
+  TSC_HIP_CHECK(se);
+  const float K[3] = {4.f, 4.f, 32.f};
+  for (int kind = 0; kind < 3; ++kind) {
+    int ok[2] = {1, 1};
+    for (int lane = 0; lane < 64; ++lane)
+      for (int r = 0; r < 4; ++r) {
+        const float row = h[kind * 512 + lane * 4 + r] / K[kind];
+        const float col = h[kind * 512 + 256 + lane * 4 + r] / K[kind];
+        if (col != (float)(lane & 15)) ok[0] = ok[1] = 0;
+        if (row != (float)(4 * (lane >> 4) + r)) ok[0] = 0;
+        if (row != (float)((lane >> 4) + 4 * r)) ok[1] = 0;
+      }
+    if (ok[0])
+      modes[kind] = 0;
+    else if (ok[1])
+      modes[kind] = 1;
+    else {
+      set_error("MFMA accumulator layout probe: unknown layout for kind " + std::to_string(kind));
+      return TSC_EHIP;
+    }
+  }
+  return TSC_OK;
+}
+
 hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_blocks,
                           const tsc_contrib *d_contribs, const Tile *d_tiles, int ntiles,
-                          hipStream_t stream) {
+                          hipStream_t stream, const int *modes) {
   if (ntiles <= 0) return hipSuccess;
   dim3 grid(ntiles), block(256);
   if (prec == TSC_PREC_F64) {
     hipLaunchKernelGGL((grouped_kernel<double, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs,
-                       d_tiles);
+                       d_tiles, modes[0]);
   } else if (prec == TSC_PREC_F32) {
     hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs,
-                       d_tiles);
+                       d_tiles, modes[1]);
   } else {
-    hipLaunchKernelGGL(grouped_kernel_bf16x3, grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles);
+    hipLaunchKernelGGL(grouped_kernel_bf16x3, grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
+                       modes[2]);
   }
   return hipGetLastError();
 }

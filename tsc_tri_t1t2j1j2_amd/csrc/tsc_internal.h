@@ -45,7 +45,9 @@ struct Bufs {
 // prec: TSC_PREC_*; tiles are for the 64x64 tile shape.
 hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_blocks,
                           const tsc_contrib *d_contribs, const Tile *d_tiles, int ntiles,
-                          hipStream_t stream);
+                          hipStream_t stream, const int *modes);
+// Determine the accumulator row layout of the f64 / f32 / bf16 MFMAs.
+int probe_layouts(hipStream_t stream, int modes[3]);
 
 // Tile shape of the grouped contraction kernels.
 constexpr int kTM = 64;
@@ -65,6 +67,7 @@ int fill_gaussian(tsc_ctx *ctx, double2 *out, int64_t n, uint64_t seed, uint64_t
 struct tsc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  int mfma_rowmode[3] = {0, 0, 0};  // accumulator layouts (f64, f32, bf16)
   double *d_red = nullptr;   // reduction partials (kRedBlocks * kMaxVecs doubles)
   double *h_red = nullptr;   // pinned host mirror of d_red
   void **d_ptrs = nullptr;   // device copy of vector pointer lists

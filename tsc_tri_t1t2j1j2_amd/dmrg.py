@@ -1,0 +1,245 @@
+"""Two-site SU(2) DMRG driver on the device hot path.
+
+Reproduces the step order of hyx15903/TSC_Tri_t1t2J1J2
+DMRG_code/dmrg_sub.f90 (warm_up_point :974-1147, left_to_right :1151-1302,
+right_to_left :1306-1465) and the schedule of DMRG_code/main.f90:114-209:
+per step basis -> super basis -> start vector -> Lanczos -> rho ->
+truncation -> renormalisation. Every numerical stage runs through the
+C-ABI (grouped contraction plans, device Lanczos, device rho/eigensolve,
+renormalisation plans); this module only orders the steps and keeps the
+blocks. Start vectors are seeded uniform random numbers (wave_initialize,
+block.f90:1154-1180, made reproducible); the guess transformation of the
+reference (wave_trans_LR/RL) is not used, each Lanczos starts afresh.
+"""
+import time
+
+import numpy as np
+
+from . import capi
+from . import hamiltonian as ham
+from . import lattice
+from .blocks import SITE_SECTORS, TotalBlock, TotalModel, site_operators
+
+
+def site_model(i):
+    """Block of one site (Get_hamiltonian_site, dmrg_sub.f90:913-942)."""
+    H = TotalBlock(0, 0, {(0, 0): np.zeros((1, 1), complex), (1, 1): np.zeros((1, 1), complex)})
+    return TotalModel([i], SITE_SECTORS, H, {i: site_operators()})
+
+
+def start_vector(super_basis, seed):
+    """Uniform [0,1) real and imaginary parts per sector, normalised."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for (x, y, dx, dy) in super_basis.sub:
+        re = rng.random((dx, dy))
+        im = rng.random((dx, dy))
+        parts.append((re + 1j * im).ravel(order="F"))
+    v = np.concatenate(parts) if parts else np.zeros(0, complex)
+    return v / np.linalg.norm(v)
+
+
+class DMRG:
+    def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
+                 kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
+                 prec=capi.PREC_F64, refine=0, seed=1234, log=None):
+        self.ctx = ctx
+        self.Lx, self.Ny = Lx, Ny
+        self.N = Lx * Ny
+        self.t1, self.t2, self.J1 = t1, t2, J1
+        self.J2 = J1 * (t2 / t1) ** 2 if J2 is None else J2
+        self.Ne = lattice.target_electrons(self.N, delta) if Ne is None else Ne
+        self.S2 = S2
+        self.kept_min, self.kept_max = kept_min, kept_max
+        self.kept_maxf = kept_max if kept_maxf is None else kept_maxf
+        self.sweeps = sweeps
+        self.lanzero = lanzero
+        self.trun_err = trun_err
+        self.prec, self.refine = prec, refine
+        self.seed = seed
+        self.bonds = lattice.bonds(Lx, Ny, t1, t2, J1, self.J2)
+        self.nb = {}
+        for (a, b, t, J) in self.bonds:
+            self.nb.setdefault(a, []).append((b, t, J))
+            self.nb.setdefault(b, []).append((a, t, J))
+        self.sys_blocks = {}
+        self.env_blocks = {}
+        self.history = []
+        self.log = log
+        self.nstep = 0
+        self.hpsi_flops = 0.0
+        self.timers = {"plan": 0.0, "lanczos": 0.0, "truncate": 0.0, "renorm": 0.0}
+
+    # ---- helpers ---------------------------------------------------------
+    def _say(self, msg):
+        if self.log:
+            self.log(msg)
+
+    def _keep_sites(self, sites_in_block, extra_partners=()):
+        inside = set(sites_in_block)
+        keep = []
+        for i in sites_in_block:
+            partners = [j for (j, _, _) in self.nb.get(i, [])] + [j for (a, j) in extra_partners if a == i]
+            if any(j not in inside for j in partners):
+                keep.append(i)
+        return keep
+
+    def _truncate(self, sb, psi_vec, side, kept_min, kept_max, trun_err, identity):
+        """U per enlarged sector and the new sector order of the grown block."""
+        basis = sb.sys_bs if side == 0 else sb.env_bs
+        if identity:
+            U = {x: np.eye(d, dtype=complex) for x, (_, _, d, _) in enumerate(basis.sectors)}
+            return U, list(range(basis.num)), 0.0, basis.dim
+        if self.S2 != 0:
+            raise NotImplementedError("truncation implemented for total spin 0 targets")
+        sd = [dx for (_, _, dx, _) in sb.super.sub]
+        ed = [dy for (_, _, _, dy) in sb.super.sub]
+        xs = [x if side == 0 else y for (x, y, _, _) in sb.super.sub]
+        twos = [basis.sectors[x][1] for x in xs]
+        t0 = time.perf_counter()
+        res = capi.rho_truncate(self.ctx, psi_vec, sd, ed, twos, side, kept_min, kept_max, trun_err)
+        self.timers["truncate"] += time.perf_counter() - t0
+        U, order = {}, []
+        for x, u, k in zip(xs, res["U"], res["kept"]):
+            if k > 0:
+                U[x] = u
+                order.append(x)
+        return U, order, res["trun_err"], res["num_state"]
+
+    def step(self, sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys=False,
+             identity_env=False, partners=()):
+        """One DMRG step; grow: 'sys', 'env' or 'both'. Returns the energy and
+        the grown block(s)."""
+        t0 = time.perf_counter()
+        sb = ham.Superblock(sys, env, s1, s2, bonds, ntot, self.S2)
+        pb = sb.plan_builder()
+        plan = pb.build(self.ctx, sb.super.dim, sb.super.dim)
+        self.timers["plan"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        psi = self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
+        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=100, max_outer=4, prec=self.prec,
+                                   refine_iters=self.refine)
+        self.timers["lanczos"] += time.perf_counter() - t0
+        self.hpsi_flops += nmv * plan.flops()
+        out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
+        if grow in ("sys", "both"):
+            U, order, err, ns = self._truncate(sb, psi, 0, self.kept_min, kept_max, trun_err, identity_sys)
+            t0 = time.perf_counter()
+            to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(s1, sys.sites, bonds)]
+            keep = self._keep_sites(sys.sites + [s1], partners)
+            out["sys"] = ham.renormalize(self.ctx, sys, s1, to_new, keep, U, order)
+            self.timers["renorm"] += time.perf_counter() - t0
+            out["sys_err"], out["sys_kept"] = err, ns
+        if grow in ("env", "both"):
+            U, order, err, ns = self._truncate(sb, psi, 1, self.kept_min, kept_max, trun_err, identity_env)
+            t0 = time.perf_counter()
+            to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(s2, env.sites, bonds)]
+            keep = self._keep_sites(env.sites + [s2], partners)
+            out["env"] = ham.renormalize(self.ctx, env, s2, to_new, keep, U, order)
+            self.timers["renorm"] += time.perf_counter() - t0
+            out["env_err"], out["env_kept"] = err, ns
+        plan.close()
+        psi.close()
+        self.nstep += 1
+        return out
+
+    @staticmethod
+    def _pairs_with(site, block_sites, bonds):
+        bs = set(block_sites)
+        out = []
+        for (a, b, t, J) in bonds:
+            if a == site and b in bs:
+                out.append((b, site, t, J))
+            elif b == site and a in bs:
+                out.append((a, site, t, J))
+        return out
+
+    # ---- warm-up (infinite algorithm) ---------------------------------
+    def warm_up(self, end_len=None):
+        N, Ny = self.N, self.Ny
+        end_len = N // 2 if end_len is None else end_len
+        sys, env = site_model(1), site_model(N)
+        self.sys_blocks[1], self.env_blocks[1] = sys, env
+        label = 1
+        tol = self.lanzero * 1e3
+        while label + 1 <= end_len:
+            l = label
+            nn = 2 * l + 2
+            nxc = (nn - 1) // Ny + 1
+            if nxc == 1:
+                nxc = 2
+            tot = Ny * nxc
+            present_small = set(range(1, l + 2)) | set(range(tot - l, tot + 1))
+            small = lattice.bonds(nxc, Ny, self.t1, self.t2, self.J1, self.J2, present=present_small)
+
+            def full(k):
+                return k if k <= l + 1 else N - tot + k
+
+            bonds = [(min(full(a), full(b)), max(full(a), full(b)), t, J) for (a, b, t, J) in small]
+            partners = [(full(a), full(b)) for (a, b, _, _) in small] + [(full(b), full(a)) for (a, b, _, _) in small]
+            ntot = int((2 * l + 2) / N * self.Ne + 0.001)
+            ntot -= ntot % 2
+            ntot = min(ntot, self.Ne)
+            s1, s2 = l + 1, N - l
+            basis_dim_sys = sum(d for (_, _, d) in sys.sectors) * 2
+            basis_dim_env = sum(d for (_, _, d) in env.sectors) * 2
+            from .blocks import TotalBasis
+            ident_s = TotalBasis(sys.sectors).dim <= self.kept_max
+            ident_e = TotalBasis(env.sectors).dim <= self.kept_max
+            del basis_dim_sys, basis_dim_env
+            r = self.step(sys, env, s1, s2, bonds, ntot, tol, "both", self.kept_max, self.trun_err * 1e2,
+                          identity_sys=ident_s, identity_env=ident_e, partners=partners)
+            sys, env = r["sys"], r["env"]
+            self.sys_blocks[l + 1], self.env_blocks[l + 1] = sys, env
+            self.history.append(("warmup", l, r["E"], r.get("sys_err", 0.0), r.get("sys_kept", 0), r["dim"]))
+            self._say(f"warm-up l={l} N_e={ntot} E={r['E']:.12f} dim={r['dim']} mv={r['matvecs']}")
+            label += 1
+
+    # ---- finite sweeps -------------------------------------------------
+    def _sweep_step_sys(self, l, tol, kept_max, trun_err):
+        N = self.N
+        sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
+        r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err)
+        self.sys_blocks[l + 1] = r["sys"]
+        return r
+
+    def _sweep_step_env(self, l, tol, kept_max, trun_err):
+        N = self.N
+        sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
+        r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "env", kept_max, trun_err)
+        self.env_blocks[N - l - 1] = r["env"]
+        return r
+
+    def left_to_right(self, l_from, l_to, tol, kept_max, trun_err, tag):
+        for l in range(l_from, l_to + 1):
+            r = self._sweep_step_sys(l, tol, kept_max, trun_err)
+            self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
+            self._say(f"{tag} L->R l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
+                      f"dim={r['dim']} mv={r['matvecs']}")
+
+    def right_to_left(self, l_from, l_to, tol, kept_max, trun_err, tag):
+        for l in range(l_from, l_to - 1, -1):
+            r = self._sweep_step_env(l, tol, kept_max, trun_err)
+            self.history.append((tag, l, r["E"], r["env_err"], r["env_kept"], r["dim"]))
+            self._say(f"{tag} R->L l={l} E={r['E']:.12f} err={r['env_err']:.3e} kept={r['env_kept']} "
+                      f"dim={r['dim']} mv={r['matvecs']}")
+
+    def run(self):
+        """Get_Ground_State (main.f90:114-209) without the final measurement
+        half-sweep; returns the last energy."""
+        N = self.N
+        self.warm_up()
+        lanerr = self.lanzero
+        tol, terr = lanerr * 15, self.trun_err * 15
+        self.left_to_right(N // 2 - 1, N - 3, tol, self.kept_max, terr, "half")
+        self.right_to_left(N - 3, 1, tol, self.kept_max, terr, "half")
+        kept_max = self.kept_max
+        for it in range(1, self.sweeps + 1):
+            if it == 1:
+                kept_max = (self.kept_max + self.kept_maxf) // 2
+            if it == 2:
+                kept_max = self.kept_maxf
+            tol = lanerr * (0.5 if it >= 5 else 1.0)
+            self.left_to_right(1, N - 3, tol, kept_max, self.trun_err, f"sweep{it}")
+            self.right_to_left(N - 3, 1, tol, kept_max, self.trun_err, f"sweep{it}")
+        return self.history[-1][2]
