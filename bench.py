@@ -32,15 +32,57 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="synthetic400")
+    ap.add_argument("--workload", default="yc6-m6000", choices=["yc6-m6000", "synthetic400", "synthetic-small"])
+    ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--lanczos-iters", type=int, default=40)
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "f32", "f64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
 
+class PhysicsWorkload:
+    """Centre superblock of a YC cylinder at M multiplets: the real term
+    list and sector structure of one DMRG step, seeded random blocks."""
+
+    def __init__(self, Lx, Ny, M, seed=7):
+        from tsc_tri_t1t2j1j2_amd import hamiltonian as ham
+        from tsc_tri_t1t2j1j2_amd import workloads
+        t0 = time.perf_counter()
+        sysb, envb, s1, s2, bonds, ntot = workloads.centre_superblock(Lx, Ny, M, seed=seed)
+        self.sb = ham.Superblock(sysb, envb, s1, s2, bonds, ntot, 0)
+        self.pb = self.sb.plan_builder()
+        self.c1, self.o1, self.c2, self.o2 = self.pb.arrays()
+        self.ops = self.pb.arena_array()
+        self.psi_len = self.sb.super.dim
+        self.N = Lx * Ny
+        self.plan_seconds = time.perf_counter() - t0
+
+    def stage_flops(self):
+        def fl(c, o):
+            f = 0.0
+            for ob in o:
+                m, n = float(ob["m"]), float(ob["n"])
+                for x in c[ob["cbeg"]:ob["cend"]]:
+                    f += 8.0 * m * n * (x["k"] if x["a_op"] != 4 and x["b_op"] != 4 else 1.0)
+            return f
+        return fl(self.c1, self.o1), fl(self.c2, self.o2)
+
+    def make_plan(self, ctx):
+        from tsc_tri_t1t2j1j2_amd import capi
+        return capi.Plan(ctx, self.ops, self.ops.size, self.pb.mid_len, self.c1, self.o1, self.c2, self.o2,
+                         self.psi_len, self.psi_len)
+
+
 def build_workload(name):
     from tsc_tri_t1t2j1j2_amd.synthetic import SyntheticSuperblock
+    if name == "yc6-m6000":
+        wl = PhysicsWorkload(24, 6, 6000)
+        desc = {"workload": "YC6 6x24 (144 sites, 132 electrons, S=0) centre superblock at M=6000 SU(2) "
+                            "multiplets: real t1-t2-J1-J2 term list and sector structure, seeded random blocks",
+                "model": "t1=1 t2=0.2 J1=0.4 J2=0.016", "psi_multiplet_dim": wl.psi_len,
+                "sectors": wl.sb.super.num, "plan_build_s": round(wl.plan_seconds, 2)}
+        return wl, desc
     if name == "synthetic400":
         sb = SyntheticSuperblock(nsec=400, nterms=24, dmin=8, dmax=1536)
         desc = {"workload": "synthetic superblock H.psi: 400 SU(2) sectors, d_L,d_R log-uniform [8,1536], "
@@ -97,6 +139,45 @@ def time_gpu(ctx, plan, vin, vout, prec, steps, warmup):
     return wall, float(np.mean(ev1)), float(np.mean(ev2))
 
 
+def lanczos_and_step(ctx, plan, wl, vin, prec, args):
+    """FP64 vs mixed Lanczos energies per site on the same superblock and the
+    device time of one DMRG step's hot path (Lanczos, rho/eigensolve/
+    truncation); sweep time extrapolated from it."""
+    from tsc_tri_t1t2j1j2_amd import capi
+    it = args.lanczos_iters
+    psi64 = ctx.vec(wl.psi_len)
+    psi64.copy_from(vin)
+    ctx.sync()
+    t0 = time.perf_counter()
+    e64, (mv64, _) = plan.lanczos(psi64, tol=1e-13, max_inner=it, max_outer=0, prec=capi.PREC_F64)
+    t64 = time.perf_counter() - t0
+    psim = ctx.vec(wl.psi_len)
+    psim.copy_from(vin)
+    ctx.sync()
+    t0 = time.perf_counter()
+    em, (mvm, _) = plan.lanczos(psim, tol=1e-13, max_inner=it, max_outer=0, prec=prec, refine_iters=8)
+    tm = time.perf_counter() - t0
+    print(f"# lanczos f64 {t64:.2f}s ({mv64} H.psi) E={e64:.12f}; mixed {tm:.2f}s ({mvm} H.psi) E={em:.12f}",
+          file=sys.stderr, flush=True)
+    sbk = wl.sb
+    sd = [dx for (_, _, dx, _) in sbk.super.sub]
+    ed = [dy for (_, _, _, dy) in sbk.super.sub]
+    twos = [sbk.sys_bs.sectors[x][1] for (x, _, _, _) in sbk.super.sub]
+    ctx.sync()
+    t0 = time.perf_counter()
+    tr = capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, 6000, 1e-15)
+    ttr = time.perf_counter() - t0
+    step = tm + ttr
+    nsteps = 2 * (wl.N - 3)
+    return {"lanczos_iters": it, "E_per_site_f64": e64 / wl.N, "E_per_site_mixed_refined": em / wl.N,
+            "dE_per_site_mixed_vs_f64": abs(em - e64) / wl.N, "lanczos_f64_s": round(t64, 3),
+            "lanczos_mixed_s": round(tm, 3), "rho_truncate_s": round(ttr, 3), "kept": int(tr["num_state"]),
+            "step_s_mixed": round(step, 3),
+            "sweep_s_estimate": round(step * nsteps, 1),
+            "sweep_estimate_note": f"{nsteps} centre-sized steps x (Lanczos {it} it + rho/truncation); "
+                                   "plan build and renormalisation excluded"}
+
+
 def cpu_baseline(sb, psi_host, seconds):
     """numpy oracle on the first (t, s) pairs until ~`seconds` of work."""
     try:
@@ -143,9 +224,9 @@ def main():
     sb, desc = build_workload(args.workload)
     t_setup = time.perf_counter()
     ctx = capi.Context(0)
-    plan = sb.make_plan(ctx, None)
+    plan = sb.make_plan(ctx) if isinstance(sb, PhysicsWorkload) else sb.make_plan(ctx, None)
     vin = ctx.vec(sb.psi_len)
-    vin.fill_gaussian(sb.seed, 0xC0FFEE, 1.0)
+    vin.fill_gaussian(getattr(sb, "seed", 20261203), 0xC0FFEE, 1.0)
     nrm = vin.norm()
     tmp = ctx.vec(sb.psi_len)
     tmp.axpby(1.0 / nrm, vin, 0.0)
@@ -154,8 +235,12 @@ def main():
     vref = ctx.vec(sb.psi_len)
     print(f"# setup {time.perf_counter() - t_setup:.1f} s, psi {sb.psi_len} complex, ops {sb.ops_len} complex",
           file=sys.stderr, flush=True)
-    F = sb.algorithmic_flops()
-    f1, f2 = stage_flops(sb)
+    if isinstance(sb, PhysicsWorkload):
+        f1, f2 = sb.stage_flops()
+        F = f1 + f2
+    else:
+        F = sb.algorithmic_flops()
+        f1, f2 = stage_flops(sb)
 
     # FP64 reference application and timing
     w64, k1_64, k2_64 = time_gpu(ctx, plan, vin, vref, capi.PREC_F64, args.steps, args.warmup)
@@ -181,10 +266,27 @@ def main():
             "frac": round(achieved / peak, 5), "traffic": None, "kernel": dom,
             "kernel_ms": round(kms, 3), "flops_per_launch": kf}
 
+    extra = {}
+    if isinstance(sb, PhysicsWorkload) and not args.no_lanczos:
+        extra = lanczos_and_step(ctx, plan, sb, vin, prec, args)
     cpu = None
     if not args.no_cpu:
         psi_host = vin.download()
-        cpu = cpu_baseline(sb, psi_host, args.cpu_seconds)
+        if isinstance(sb, PhysicsWorkload):
+            from oracle import plan_exec
+            try:
+                from threadpoolctl import threadpool_info
+                threads = max([d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"]
+                              or [1])
+            except Exception:
+                threads = os.cpu_count() or 1
+            fl, el, nw = plan_exec.timed_sample(sb.ops, sb.c1, sb.o1, sb.c2, sb.o2, sb.pb.mid_len, psi_host,
+                                                args.cpu_seconds)
+            cpu = {"value": fl / el / 1e12, "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
+                   "sample": f"numpy/OpenBLAS execution of {nw} random output windows of the same H.psi plan "
+                             f"(with their intermediates), {fl / 1e12:.3f} TFLOP in {el:.1f} s"}
+        else:
+            cpu = cpu_baseline(sb, psi_host, args.cpu_seconds)
 
     line = {
         "metric": "sweep time + H\u00b7\u03c8 TFLOP/s, YC6 M=6000: mixed vs FP64 GPU at |\u0394E|/site\u22641e-8",
@@ -205,6 +307,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    line["config"].update(extra)
     print(json.dumps(line), flush=True)
 
 

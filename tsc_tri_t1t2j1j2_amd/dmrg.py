@@ -102,7 +102,10 @@ class DMRG:
         U, order = {}, []
         for x, u, k in zip(xs, res["U"], res["kept"]):
             if k > 0:
-                U[x] = u
+                # env operators act on psi as psi.O^T here, so the env reduced
+                # density matrix is conj(psi^H psi): its eigenvectors are the
+                # conjugates of those of the reference's rho (den_mat.f90:89)
+                U[x] = u if side == 0 else np.conj(u)
                 order.append(x)
         return U, order, res["trun_err"], res["num_state"]
 
