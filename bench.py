@@ -55,6 +55,7 @@ class PhysicsWorkload:
         self.c1, self.o1, self.c2, self.o2 = self.pb.arrays()
         self.ops = self.pb.arena_array()
         self.psi_len = self.sb.super.dim
+        self.ops_len = self.ops.size
         self.N = Lx * Ny
         self.plan_seconds = time.perf_counter() - t0
 
@@ -300,7 +301,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"bf16x3": "bf16x3 (fp32 accumulate)", "f32": "complex f32", "f64": "complex f64"}[args.prec],
-        "data": "synthetic (seeded counter-based Gaussian operators and psi)",
+        "data": ("synthetic: seeded random block operators with the sector structure and term list of a real "
+                 "DMRG step; random unit psi" if isinstance(sb, PhysicsWorkload)
+                 else "synthetic (seeded counter-based Gaussian operators and psi)"),
         "config": dict(desc, algorithmic_flops_per_step=F, precision=args.prec,
                        f64_ms_per_step=round(w64 * 1e3, 4), f64_tflops=round(F / w64 / 1e12, 4),
                        speedup_vs_f64=round(w64 / wall, 3), rel_err_vs_f64=rel, parallelism="1 GPU"),
