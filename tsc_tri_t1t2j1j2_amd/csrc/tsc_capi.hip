@@ -243,29 +243,72 @@ int check_stage(const tsc_contrib *c, int32_t nc, const tsc_outblock *o, int32_t
   return TSC_OK;
 }
 
-std::vector<Tile> make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no) {
+// Work units of one stage: every 64x64 tile of every output block; tiles
+// whose contribution list is long are cut into contiguous chunks of about
+// `target` k-steps that write partial tiles (summed by a Reduce entry).
+// Units are ordered by work, largest first.
+struct Units {
   std::vector<Tile> tiles;
+  std::vector<Reduce> reds;
+  int64_t nslot = 0;
+};
+
+Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no) {
+  auto ccost = [&](int32_t ci) {
+    return (c[ci].a_op == TSC_OP_I || c[ci].b_op == TSC_OP_I) ? 1.0 : (double)std::max(c[ci].k, 1);
+  };
+  double total = 0.0;
+  int64_t ntile = 0;
+  for (int32_t b = 0; b < no; ++b) {
+    if (o[b].m == 0 || o[b].n == 0) continue;
+    double w = 0.0;
+    for (int32_t ci = o[b].cbeg; ci < o[b].cend; ++ci) w += ccost(ci);
+    const int64_t nt = (int64_t)((o[b].m + kTM - 1) / kTM) * ((o[b].n + kTN - 1) / kTN);
+    total += std::max(w, 1.0) * nt;
+    ntile += nt;
+  }
+  // aim at several units per wavefront slot of the device (256 CUs x 3 WGs)
+  const double desired = 6144.0;
+  const double target = (ntile >= (int64_t)desired) ? 1e300 : std::max(total / desired, 64.0);
+  Units u;
   std::vector<double> cost;
   for (int32_t b = 0; b < no; ++b) {
     const tsc_outblock &ob = o[b];
     if (ob.m == 0 || ob.n == 0) continue;
     double w = 0.0;
-    for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci)
-      w += (c[ci].a_op == TSC_OP_I || c[ci].b_op == TSC_OP_I) ? 1.0 : (double)std::max(c[ci].k, 1);
-    w = std::max(w, 1.0);
+    for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) w += ccost(ci);
     const int nti = (ob.m + kTM - 1) / kTM, ntj = (ob.n + kTN - 1) / kTN;
     for (int ti = 0; ti < nti; ++ti)
       for (int tj = 0; tj < ntj; ++tj) {
-        tiles.push_back(Tile{b, (int16_t)ti, (int16_t)tj});
-        cost.push_back(w);
+        if (w <= target || ob.cend - ob.cbeg <= 1) {
+          u.tiles.push_back(Tile{b, (int16_t)ti, (int16_t)tj, ob.cbeg, ob.cend, -1, 0});
+          cost.push_back(std::max(w, 1.0));
+          continue;
+        }
+        Reduce r{b, (int16_t)ti, (int16_t)tj, (int32_t)u.nslot, 0};
+        int32_t start = ob.cbeg;
+        double acc = 0.0;
+        for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) {
+          acc += ccost(ci);
+          if (acc >= target || ci + 1 == ob.cend) {
+            u.tiles.push_back(Tile{b, (int16_t)ti, (int16_t)tj, start, ci + 1, (int32_t)u.nslot, 0});
+            cost.push_back(acc);
+            ++u.nslot;
+            ++r.nslot;
+            start = ci + 1;
+            acc = 0.0;
+          }
+        }
+        u.reds.push_back(r);
       }
   }
-  std::vector<size_t> order(tiles.size());
+  std::vector<size_t> order(u.tiles.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-  std::vector<Tile> sorted(tiles.size());
-  for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
-  return sorted;
+  std::vector<Tile> sorted(u.tiles.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = u.tiles[order[i]];
+  u.tiles.swap(sorted);
+  return u;
 }
 
 template <typename T>
@@ -293,7 +336,10 @@ extern "C" int tsc_plan_destroy(tsc_plan *p) {
     hipFree(p->d_c[s]);
     hipFree(p->d_o[s]);
     hipFree(p->d_t[s]);
+    hipFree(p->d_r[s]);
   }
+  hipFree(p->d_part64);
+  hipFree(p->d_part32);
   for (int i = 0; i < 3; ++i) {
     if (p->ev[i]) hipEventDestroy(p->ev[i]);
     if (p->gexec[i]) hipGraphExecDestroy(p->gexec[i]);
@@ -358,9 +404,12 @@ int plan_create_impl(tsc_ctx *ctx, const double *ops, const double2 *ops_dev, in
   for (int s = 0; s < 2; ++s) {
     if ((rc = upload<tsc_contrib>(&p->d_c[s], cs[s], ncs[s], ctx->stream))) return fail(rc);
     if ((rc = upload<tsc_outblock>(&p->d_o[s], os[s], nos[s], ctx->stream))) return fail(rc);
-    std::vector<Tile> tiles = make_tiles(cs[s], os[s], nos[s]);
-    p->nt[s] = (int)tiles.size();
-    if ((rc = upload<Tile>(&p->d_t[s], tiles.data(), tiles.size(), ctx->stream))) return fail(rc);
+    Units un = make_tiles(cs[s], os[s], nos[s]);
+    p->nt[s] = (int)un.tiles.size();
+    p->nr[s] = (int)un.reds.size();
+    p->nslot[s] = un.nslot;
+    if ((rc = upload<Tile>(&p->d_t[s], un.tiles.data(), un.tiles.size(), ctx->stream))) return fail(rc);
+    if ((rc = upload<Reduce>(&p->d_r[s], un.reds.data(), un.reds.size(), ctx->stream))) return fail(rc);
     TSC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   }
   for (int i = 0; i < 3; ++i) TSC_HIP_CHECK(hipEventCreate(&p->ev[i]));
@@ -414,6 +463,12 @@ extern "C" int tsc_plan_flops(const tsc_plan *p, double *flops) {
 namespace {
 int ensure_precision(tsc_plan *p, int prec) {
   hipStream_t s = p->ctx->stream;
+  const int64_t slots = std::max(p->nslot[0], p->nslot[1]);
+  if (slots > 0) {
+    void **part = (prec == TSC_PREC_F64) ? &p->d_part64 : &p->d_part32;
+    const size_t esz = (prec == TSC_PREC_F64) ? sizeof(double2) : sizeof(float2);
+    if (!*part) TSC_HIP_CHECK(hipMalloc(part, esz * (size_t)kTM * kTN * (size_t)slots));
+  }
   if (prec == TSC_PREC_F64) {
     if (!p->d_mid64 && p->mid_elems > 0) TSC_HIP_CHECK(hipMalloc((void **)&p->d_mid64, sizeof(double2) * p->mid_elems));
   } else {
@@ -435,9 +490,12 @@ int enqueue_hpsi(tsc_plan *p, const double2 *in, double2 *out, int prec, bool ev
   b.p[TSC_BUF_MID] = (prec == TSC_PREC_F64) ? (const void *)p->d_mid64 : (const void *)p->d_mid32;
   b.p[TSC_BUF_OUT] = out;
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[0], s));
-  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[0], p->d_c[0], p->d_t[0], p->nt[0], s, p->ctx->mfma_rowmode));
+  void *part = (prec == TSC_PREC_F64) ? p->d_part64 : p->d_part32;
+  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[0], p->d_c[0], p->d_t[0], p->nt[0], p->d_r[0], p->nr[0], part, s,
+                               p->ctx->mfma_rowmode));
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[1], s));
-  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[1], p->d_c[1], p->d_t[1], p->nt[1], s, p->ctx->mfma_rowmode));
+  TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[1], p->d_c[1], p->d_t[1], p->nt[1], p->d_r[1], p->nr[1], part, s,
+                               p->ctx->mfma_rowmode));
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[2], s));
   return TSC_OK;
 }

@@ -1,16 +1,23 @@
 // Grouped complex contraction kernels for gfx950 (CDNA4).
 //
-// One workgroup (4 waves, 256 lanes) owns one 64x64 tile of one output
-// block and accumulates every contribution of that block in registers:
-//     C_tile = sum_c coef_c * op_a(A_c) * op_b(B_c)
-// so no atomics are needed and every output element is written once.
-// This is the batched replacement for the per-sector ZGEMM pairs of
-// model_wave's kernels (hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/get_sys.f90:
-// 1817, 1869, 2555-2560, 3385-3408, ...).
+// A work unit is one 64x64 tile of one output block together with a
+// contiguous range of that block's contributions:
+//     C_tile (+)= sum_c coef_c * op_a(A_c) * op_b(B_c)
+// Units that own their whole range write C directly; a long range may be
+// cut into several units that write partial tiles, summed afterwards in a
+// fixed order by reduce_kernel (deterministic, no atomics). This is the
+// batched replacement for the per-sector ZGEMM pairs of model_wave's
+// kernels (hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/get_sys.f90: 1817, 1869,
+// 2555-2560, 3385-3408, ...).
+//
+// The K loop runs over the flattened sequence of (contribution, k0) steps
+// of the unit; the global loads of the next step are issued before the
+// MFMAs of the current one, so memory latency hides behind the matrix
+// cores. Partial K steps issue only the MFMA k-groups they need.
 //
 // Waves are arranged 2x2 over the tile; each wave computes a 32x32 sub-tile
 // as 2x2 MFMA 16x16 blocks, real and imaginary parts in separate
-// accumulators (4 real MFMAs per complex k-step).
+// accumulators (4 real MFMAs per complex k-group).
 //   FP64  : v_mfma_f64_16x16x4f64, TK = 16
 //   FP32  : v_mfma_f32_16x16x4f32, TK = 16 (complex FP32 products)
 //   BF16x3: v_mfma_f32_16x16x32_bf16 on hi/lo splits, TK = 32
@@ -42,8 +49,8 @@ __device__ __forceinline__ void ldbuf(const Bufs &B, int buf, int64_t off, CT &r
 
 // op(X)[r, c] for a column-major X with leading dimension ld
 template <typename CT>
-__device__ __forceinline__ void ld_op(const Bufs &B, int buf, int64_t off, int ld, int op, int r,
-                                      int c, CT &re, CT &im) {
+__device__ __forceinline__ void ld_op(const Bufs &B, int buf, int64_t off, int ld, int op, int r, int c, CT &re,
+                                      CT &im) {
   const bool tr = (op == TSC_OP_T || op == TSC_OP_C);
   const int64_t idx = tr ? (off + (int64_t)c + (int64_t)r * ld) : (off + (int64_t)r + (int64_t)c * ld);
   ldbuf<CT>(B, buf, idx, re, im);
@@ -61,6 +68,99 @@ template <typename CT>
 __device__ __forceinline__ void cmul(CT ar, CT ai, CT br, CT bi, CT &cr, CT &ci) {
   cr = ar * br - ai * bi;
   ci = ar * bi + ai * br;
+}
+
+__device__ __forceinline__ bool is_elementwise(const tsc_contrib &c) {
+  return c.a_op == TSC_OP_I || c.b_op == TSC_OP_I;
+}
+
+__device__ __forceinline__ int next_gemm(const tsc_contrib *__restrict__ contribs, int ci, int cend) {
+  while (ci < cend && is_elementwise(contribs[ci])) ++ci;
+  return ci;
+}
+
+// elementwise contributions (one operand the identity) of a unit
+template <typename CT, typename ACC>
+__device__ __forceinline__ void elementwise_pass(const Bufs &bufs, const tsc_contrib *__restrict__ contribs, int cbeg,
+                                                 int cend, int i0, int j0, int m, int n, int wm, int wn, int lane,
+                                                 int rowmode, ACC (&accR)[2][2], ACC (&accI)[2][2]) {
+  for (int ci = cbeg; ci < cend; ++ci) {
+    const tsc_contrib c = contribs[ci];
+    if (!is_elementwise(c)) continue;
+    const bool useA = (c.b_op == TSC_OP_I);
+    const int buf = useA ? c.a_buf : c.b_buf;
+    const int64_t off = useA ? c.a_off : c.b_off;
+    const int ldx = useA ? c.lda : c.ldb;
+    const int op = useA ? c.a_op : c.b_op;
+    if (op == TSC_OP_I) continue;
+    const CT cre = (CT)c.coef_re, cim = (CT)c.coef_im;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
+          const int j = j0 + wn + b * 16 + (lane & 15);
+          if (i < m && j < n) {
+            CT xr, xi, pr, pi;
+            ld_op<CT>(bufs, buf, off, ldx, op, i, j, xr, xi);
+            cmul<CT>(cre, cim, xr, xi, pr, pi);
+            accR[a][b][r] += pr;
+            accI[a][b][r] += pi;
+          }
+        }
+  }
+}
+
+// write the unit's accumulators: to C, or as a partial tile to part[slot]
+template <typename CT, typename ACC>
+__device__ __forceinline__ void write_unit(const Bufs &bufs, const tsc_outblock &ob, const Tile &t, void *part,
+                                           int i0, int j0, int wm, int wn, int lane, int rowmode,
+                                           const ACC (&accR)[2][2], const ACC (&accI)[2][2]) {
+  const int m = ob.m, n = ob.n;
+  const bool out_f64 = (ob.c_buf == TSC_BUF_IN || ob.c_buf == TSC_BUF_OUT || sizeof(CT) == 8);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int il = wm + a * 16 + acc_row(rowmode, lane, r);
+        const int jl = wn + b * 16 + (lane & 15);
+        const int i = i0 + il, j = j0 + jl;
+        if (i >= m || j >= n) continue;
+        if (t.slot >= 0) {
+          if (sizeof(CT) == 8) {
+            double2 *P = reinterpret_cast<double2 *>(part) + (int64_t)t.slot * (TM * TN);
+            P[jl * TM + il] = make_double2((double)accR[a][b][r], (double)accI[a][b][r]);
+          } else {
+            float2 *P = reinterpret_cast<float2 *>(part) + (int64_t)t.slot * (TM * TN);
+            P[jl * TM + il] = make_float2((float)accR[a][b][r], (float)accI[a][b][r]);
+          }
+          continue;
+        }
+        const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
+        if (out_f64) {
+          double2 *C = reinterpret_cast<double2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+          double2 v = make_double2((double)accR[a][b][r], (double)accI[a][b][r]);
+          if (ob.mode) {
+            const double2 o = C[idx];
+            v.x += o.x;
+            v.y += o.y;
+          }
+          C[idx] = v;
+        } else {
+          float2 *C = reinterpret_cast<float2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+          float2 v = make_float2((float)accR[a][b][r], (float)accI[a][b][r]);
+          if (ob.mode) {
+            const float2 o = C[idx];
+            v.x += o.x;
+            v.y += o.y;
+          }
+          C[idx] = v;
+        }
+      }
 }
 
 // ---------------------------------------------------------------------
@@ -82,12 +182,75 @@ struct MfmaTraits<float> {
   }
 };
 
+// Per-thread staging registers of one (contribution, k0) step.
+template <typename CT, int NA, int NB>
+struct Stage {
+  CT ar[NA], ai[NA], br[NB], bi[NB];
+  CT cre, cim;
+  int nk;  // valid k of this step (<= TK)
+};
+
+// Thread mapping of the A tile (TM x TK) and B tile (TK x TN) for a step:
+// the contiguous global index runs fastest across threads.
+template <int TK>
+__device__ __forceinline__ void a_coord(bool a_tr, int tid, int r, int &ii, int &kk) {
+  if (!a_tr) {
+    ii = tid % TM;
+    kk = tid / TM + r * (256 / TM);
+  } else {
+    kk = tid % TK;
+    ii = tid / TK + r * (256 / TK);
+  }
+}
+template <int TK>
+__device__ __forceinline__ void b_coord(bool b_tr, int tid, int r, int &jj, int &kk) {
+  if (b_tr) {
+    jj = tid % TN;
+    kk = tid / TN + r * (256 / TN);
+  } else {
+    kk = tid % TK;
+    jj = tid / TK + r * (256 / TK);
+  }
+}
+
+template <typename CT, int TK, int NA, int NB>
+__device__ __forceinline__ void load_stage(const Bufs &bufs, const tsc_contrib &c, int k0, int i0, int j0, int m, int n,
+                                           int tid, Stage<CT, NA, NB> &s) {
+  const int K = c.k;
+  const bool a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
+  const bool b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
+#pragma unroll
+  for (int r = 0; r < NA; ++r) {
+    int ii, kk;
+    a_coord<TK>(a_tr, tid, r, ii, kk);
+    const int gi = i0 + ii, gk = k0 + kk;
+    CT xr = 0, xi = 0;
+    if (gi < m && gk < K) ld_op<CT>(bufs, c.a_buf, c.a_off, c.lda, c.a_op, gi, gk, xr, xi);
+    s.ar[r] = xr;
+    s.ai[r] = xi;
+  }
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    int jj, kk;
+    b_coord<TK>(b_tr, tid, r, jj, kk);
+    const int gj = j0 + jj, gk = k0 + kk;
+    CT xr = 0, xi = 0;
+    if (gj < n && gk < K) ld_op<CT>(bufs, c.b_buf, c.b_off, c.ldb, c.b_op, gk, gj, xr, xi);
+    s.br[r] = xr;
+    s.bi[r] = xi;
+  }
+  s.cre = (CT)c.coef_re;
+  s.cim = (CT)c.coef_im;
+  s.nk = min(TK, K - k0);
+}
+
 template <typename CT, int TK>
 __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outblock *__restrict__ blocks,
                                                       const tsc_contrib *__restrict__ contribs,
-                                                      const Tile *__restrict__ tiles, int rowmode) {
+                                                      const Tile *__restrict__ tiles, int rowmode, void *part) {
   typedef typename MfmaTraits<CT>::acc_t acc_t;
   constexpr int PAD = (sizeof(CT) == 8) ? 16 : 32;  // row padding (elements)
+  constexpr int NA = (TM * TK) / 256, NB = (TN * TK) / 256;
   __shared__ CT As_r[TK][TM + PAD], As_i[TK][TM + PAD];
   __shared__ CT Bs_r[TK][TN + PAD], Bs_i[TK][TN + PAD];
 
@@ -110,78 +273,57 @@ __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outbl
       accI[a][b] = acc_t{0, 0, 0, 0};
     }
 
-  for (int ci = ob.cbeg; ci < ob.cend; ++ci) {
+  elementwise_pass<CT>(bufs, contribs, t.cbeg, t.cend, i0, j0, m, n, wm, wn, lane, rowmode, accR, accI);
+
+  // flattened, software-pipelined (contribution, k0) loop
+  int ci = next_gemm(contribs, t.cbeg, t.cend);
+  int k0 = 0;
+  Stage<CT, NA, NB> st;
+  bool a_tr = false, b_tr = false;
+  if (ci < t.cend) {
     const tsc_contrib c = contribs[ci];
-    const CT cre = (CT)c.coef_re, cim = (CT)c.coef_im;
-    if (c.a_op == TSC_OP_I || c.b_op == TSC_OP_I) {
-      // elementwise term: acc(i,j) += coef * X(i,j)
-      const bool useA = (c.b_op == TSC_OP_I);
-      const int buf = useA ? c.a_buf : c.b_buf;
-      const int64_t off = useA ? c.a_off : c.b_off;
-      const int ldx = useA ? c.lda : c.ldb;
-      const int op = useA ? c.a_op : c.b_op;
-      if (op == TSC_OP_I) continue;  // I*I is not a valid term
+    load_stage<CT, TK, NA, NB>(bufs, c, 0, i0, j0, m, n, tid, st);
+    a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
+    b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
+  }
+  int Kcur = (ci < t.cend) ? contribs[ci].k : 0;
+  while (ci < t.cend) {
+    // stage the current step (A scaled by its coefficient)
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
-            const int j = j0 + wn + b * 16 + (lane & 15);
-            if (i < m && j < n) {
-              CT xr, xi, pr, pi;
-              ld_op<CT>(bufs, buf, off, ldx, op, i, j, xr, xi);
-              cmul<CT>(cre, cim, xr, xi, pr, pi);
-              accR[a][b][r] += pr;
-              accI[a][b][r] += pi;
-            }
-          }
-      continue;
+    for (int r = 0; r < NA; ++r) {
+      int ii, kk;
+      a_coord<TK>(a_tr, tid, r, ii, kk);
+      CT pr, pi;
+      cmul<CT>(st.cre, st.cim, st.ar[r], st.ai[r], pr, pi);
+      As_r[kk][ii] = pr;
+      As_i[kk][ii] = pi;
     }
-    const int K = c.k;
-    const bool a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
-    const bool b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
-    for (int k0 = 0; k0 < K; k0 += TK) {
-      // ---- stage A tile (TM x TK) scaled by coef
 #pragma unroll
-      for (int r = 0; r < (TM * TK) / 256; ++r) {
-        int ii, kk;
-        if (!a_tr) {
-          ii = tid % TM;
-          kk = tid / TM + r * (256 / TM);
-        } else {
-          kk = tid % TK;
-          ii = tid / TK + r * (256 / TK);
-        }
-        const int gi = i0 + ii, gk = k0 + kk;
-        CT xr = 0, xi = 0;
-        if (gi < m && gk < K) ld_op<CT>(bufs, c.a_buf, c.a_off, c.lda, c.a_op, gi, gk, xr, xi);
-        CT pr, pi;
-        cmul<CT>(cre, cim, xr, xi, pr, pi);
-        As_r[kk][ii] = pr;
-        As_i[kk][ii] = pi;
-      }
-      // ---- stage B tile (TK x TN)
+    for (int r = 0; r < NB; ++r) {
+      int jj, kk;
+      b_coord<TK>(b_tr, tid, r, jj, kk);
+      Bs_r[kk][jj] = st.br[r];
+      Bs_i[kk][jj] = st.bi[r];
+    }
+    const int nks = (st.nk + 3) >> 2;
+    __syncthreads();
+    // issue the next step's loads
+    int nci = ci, nk0 = k0 + TK;
+    if (nk0 >= Kcur) {
+      nci = next_gemm(contribs, ci + 1, t.cend);
+      nk0 = 0;
+    }
+    if (nci < t.cend) {
+      const tsc_contrib c = contribs[nci];
+      load_stage<CT, TK, NA, NB>(bufs, c, nk0, i0, j0, m, n, tid, st);
+      a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
+      b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
+      if (nci != ci) Kcur = c.k;
+    }
+    // MFMAs of the current step
 #pragma unroll
-      for (int r = 0; r < (TN * TK) / 256; ++r) {
-        int jj, kk;
-        if (b_tr) {
-          jj = tid % TN;
-          kk = tid / TN + r * (256 / TN);
-        } else {
-          kk = tid % TK;
-          jj = tid / TK + r * (256 / TK);
-        }
-        const int gj = j0 + jj, gk = k0 + kk;
-        CT xr = 0, xi = 0;
-        if (gj < n && gk < K) ld_op<CT>(bufs, c.b_buf, c.b_off, c.ldb, c.b_op, gk, gj, xr, xi);
-        Bs_r[kk][jj] = xr;
-        Bs_i[kk][jj] = xi;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int ks = 0; ks < TK / 4; ++ks) {
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      if (ks < nks) {
         const int kr = ks * 4 + (lane >> 4);
         CT ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
@@ -205,43 +347,13 @@ __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outbl
             accI[a][b] = MfmaTraits<CT>::mma(ai[a], br[b], accI[a][b]);
           }
       }
-      __syncthreads();
     }
+    __syncthreads();
+    ci = nci;
+    k0 = nk0;
   }
 
-  // ---- write the tile
-  const bool out_f64 = (ob.c_buf == TSC_BUF_IN || ob.c_buf == TSC_BUF_OUT || sizeof(CT) == 8);
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
-        const int j = j0 + wn + b * 16 + (lane & 15);
-        if (i < m && j < n) {
-          const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
-          if (out_f64) {
-            double2 *C = reinterpret_cast<double2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
-            double2 v = make_double2((double)accR[a][b][r], (double)accI[a][b][r]);
-            if (ob.mode) {
-              const double2 o = C[idx];
-              v.x += o.x;
-              v.y += o.y;
-            }
-            C[idx] = v;
-          } else {
-            float2 *C = reinterpret_cast<float2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
-            float2 v = make_float2((float)accR[a][b][r], (float)accI[a][b][r]);
-            if (ob.mode) {
-              const float2 o = C[idx];
-              v.x += o.x;
-              v.y += o.y;
-            }
-            C[idx] = v;
-          }
-        }
-      }
+  write_unit<CT>(bufs, ob, t, part, i0, j0, wm, wn, lane, rowmode, accR, accI);
 }
 
 // ---------------------------------------------------------------------
@@ -253,8 +365,10 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
 
 __global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const tsc_outblock *__restrict__ blocks,
                                                              const tsc_contrib *__restrict__ contribs,
-                                                             const Tile *__restrict__ tiles, int rowmode) {
+                                                             const Tile *__restrict__ tiles, int rowmode,
+                                                             void *part) {
   constexpr int TK = 32;
+  constexpr int NA = (TM * TK) / 256, NB = (TN * TK) / 256;
   // layout [row][k] so that each lane reads 8 consecutive k (16 bytes)
   constexpr int KP = TK + 8;
   __shared__ __bf16 Ah_r[TM][KP], Al_r[TM][KP], Ah_i[TM][KP], Al_i[TM][KP];
@@ -279,152 +393,147 @@ __global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const ts
       accI[a][b] = f4{0, 0, 0, 0};
     }
 
-  for (int ci = ob.cbeg; ci < ob.cend; ++ci) {
+  elementwise_pass<float>(bufs, contribs, t.cbeg, t.cend, i0, j0, m, n, wm, wn, lane, rowmode, accR, accI);
+
+  int ci = next_gemm(contribs, t.cbeg, t.cend);
+  int k0 = 0;
+  Stage<float, NA, NB> st;
+  bool a_tr = false, b_tr = false;
+  if (ci < t.cend) {
     const tsc_contrib c = contribs[ci];
-    const float cre = (float)c.coef_re, cim = (float)c.coef_im;
-    if (c.a_op == TSC_OP_I || c.b_op == TSC_OP_I) {
-      const bool useA = (c.b_op == TSC_OP_I);
-      const int buf = useA ? c.a_buf : c.b_buf;
-      const int64_t off = useA ? c.a_off : c.b_off;
-      const int ldx = useA ? c.lda : c.ldb;
-      const int op = useA ? c.a_op : c.b_op;
-      if (op == TSC_OP_I) continue;
+    load_stage<float, TK, NA, NB>(bufs, c, 0, i0, j0, m, n, tid, st);
+    a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
+    b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
+  }
+  int Kcur = (ci < t.cend) ? contribs[ci].k : 0;
+  while (ci < t.cend) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
-            const int j = j0 + wn + b * 16 + (lane & 15);
-            if (i < m && j < n) {
-              float xr, xi, pr, pi;
-              ld_op<float>(bufs, buf, off, ldx, op, i, j, xr, xi);
-              cmul<float>(cre, cim, xr, xi, pr, pi);
-              accR[a][b][r] += pr;
-              accI[a][b][r] += pi;
-            }
-          }
-      continue;
+    for (int r = 0; r < NA; ++r) {
+      int ii, kk;
+      a_coord<TK>(a_tr, tid, r, ii, kk);
+      float pr, pi;
+      cmul<float>(st.cre, st.cim, st.ar[r], st.ai[r], pr, pi);
+      split_bf16(pr, Ah_r[ii][kk], Al_r[ii][kk]);
+      split_bf16(pi, Ah_i[ii][kk], Al_i[ii][kk]);
     }
-    const int K = c.k;
-    const bool a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
-    const bool b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
-    for (int k0 = 0; k0 < K; k0 += TK) {
 #pragma unroll
-      for (int r = 0; r < (TM * TK) / 256; ++r) {
-        int ii, kk;
-        if (!a_tr) {
-          ii = tid % TM;
-          kk = tid / TM + r * (256 / TM);
-        } else {
-          kk = tid % TK;
-          ii = tid / TK + r * (256 / TK);
-        }
-        const int gi = i0 + ii, gk = k0 + kk;
-        float xr = 0, xi = 0;
-        if (gi < m && gk < K) ld_op<float>(bufs, c.a_buf, c.a_off, c.lda, c.a_op, gi, gk, xr, xi);
-        float pr, pi;
-        cmul<float>(cre, cim, xr, xi, pr, pi);
-        split_bf16(pr, Ah_r[ii][kk], Al_r[ii][kk]);
-        split_bf16(pi, Ah_i[ii][kk], Al_i[ii][kk]);
-      }
+    for (int r = 0; r < NB; ++r) {
+      int jj, kk;
+      b_coord<TK>(b_tr, tid, r, jj, kk);
+      split_bf16(st.br[r], Bh_r[jj][kk], Bl_r[jj][kk]);
+      split_bf16(st.bi[r], Bh_i[jj][kk], Bl_i[jj][kk]);
+    }
+    __syncthreads();
+    int nci = ci, nk0 = k0 + TK;
+    if (nk0 >= Kcur) {
+      nci = next_gemm(contribs, ci + 1, t.cend);
+      nk0 = 0;
+    }
+    if (nci < t.cend) {
+      const tsc_contrib c = contribs[nci];
+      load_stage<float, TK, NA, NB>(bufs, c, nk0, i0, j0, m, n, tid, st);
+      a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
+      b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
+      if (nci != ci) Kcur = c.k;
+    }
+    // 16x16x32 bf16: lane holds row (lane&15), k = 8*(lane>>4) + 0..7
+    const int kq = 8 * (lane >> 4);
+    bf8 ahr[2], alr[2], ahi[2], ali[2], nahi[2], nali[2], bhr[2], blr[2], bhi[2], bli[2];
 #pragma unroll
-      for (int r = 0; r < (TN * TK) / 256; ++r) {
-        int jj, kk;
-        if (b_tr) {
-          jj = tid % TN;
-          kk = tid / TN + r * (256 / TN);
-        } else {
-          kk = tid % TK;
-          jj = tid / TK + r * (256 / TK);
-        }
-        const int gj = j0 + jj, gk = k0 + kk;
-        float xr = 0, xi = 0;
-        if (gj < n && gk < K) ld_op<float>(bufs, c.b_buf, c.b_off, c.ldb, c.b_op, gk, gj, xr, xi);
-        split_bf16(xr, Bh_r[jj][kk], Bl_r[jj][kk]);
-        split_bf16(xi, Bh_i[jj][kk], Bl_i[jj][kk]);
-      }
-      __syncthreads();
-      // 16x16x32 bf16: lane holds row (lane&15), k = 8*(lane>>4) + 0..7
-      const int kq = 8 * (lane >> 4);
-      bf8 ahr[2], alr[2], ahi[2], ali[2], nahi[2], nali[2], bhr[2], blr[2], bhi[2], bli[2];
+    for (int a = 0; a < 2; ++a) {
+      const int row = wm + a * 16 + (lane & 15);
+      ahr[a] = *reinterpret_cast<const bf8 *>(&Ah_r[row][kq]);
+      alr[a] = *reinterpret_cast<const bf8 *>(&Al_r[row][kq]);
+      ahi[a] = *reinterpret_cast<const bf8 *>(&Ah_i[row][kq]);
+      ali[a] = *reinterpret_cast<const bf8 *>(&Al_i[row][kq]);
+      nahi[a] = -ahi[a];
+      nali[a] = -ali[a];
+    }
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const int row = wm + a * 16 + (lane & 15);
-        ahr[a] = *reinterpret_cast<const bf8 *>(&Ah_r[row][kq]);
-        alr[a] = *reinterpret_cast<const bf8 *>(&Al_r[row][kq]);
-        ahi[a] = *reinterpret_cast<const bf8 *>(&Ah_i[row][kq]);
-        ali[a] = *reinterpret_cast<const bf8 *>(&Al_i[row][kq]);
-        nahi[a] = -ahi[a];
-        nali[a] = -ali[a];
-      }
+    for (int b = 0; b < 2; ++b) {
+      const int col = wn + b * 16 + (lane & 15);
+      bhr[b] = *reinterpret_cast<const bf8 *>(&Bh_r[col][kq]);
+      blr[b] = *reinterpret_cast<const bf8 *>(&Bl_r[col][kq]);
+      bhi[b] = *reinterpret_cast<const bf8 *>(&Bh_i[col][kq]);
+      bli[b] = *reinterpret_cast<const bf8 *>(&Bl_i[col][kq]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const int col = wn + b * 16 + (lane & 15);
-        bhr[b] = *reinterpret_cast<const bf8 *>(&Bh_r[col][kq]);
-        blr[b] = *reinterpret_cast<const bf8 *>(&Bl_r[col][kq]);
-        bhi[b] = *reinterpret_cast<const bf8 *>(&Bh_i[col][kq]);
-        bli[b] = *reinterpret_cast<const bf8 *>(&Bl_i[col][kq]);
+        f4 r = accR[a][b], i = accI[a][b];
+        // Re += ar*br - ai*bi  (3-term split each)
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhr[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], blr[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nali[a], bhi[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bli[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhr[b], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bhi[b], r, 0, 0, 0);
+        // Im += ar*bi + ai*br
+        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhi[b], i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bli[b], i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ali[a], bhr[b], i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], blr[b], i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhi[b], i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], bhr[b], i, 0, 0, 0);
+        accR[a][b] = r;
+        accI[a][b] = i;
       }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          f4 r = accR[a][b], i = accI[a][b];
-          // Re += ar*br - ai*bi  (3-term split each)
-          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhr[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], blr[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nali[a], bhi[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bli[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhr[b], r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bhi[b], r, 0, 0, 0);
-          // Im += ar*bi + ai*br
-          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhi[b], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bli[b], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ali[a], bhr[b], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], blr[b], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhi[b], i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], bhr[b], i, 0, 0, 0);
-          accR[a][b] = r;
-          accI[a][b] = i;
-        }
-      __syncthreads();
-    }
+    __syncthreads();
+    ci = nci;
+    k0 = nk0;
   }
 
-  const bool out_f64 = (ob.c_buf == TSC_BUF_IN || ob.c_buf == TSC_BUF_OUT);
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = i0 + wm + a * 16 + acc_row(rowmode, lane, r);
-        const int j = j0 + wn + b * 16 + (lane & 15);
-        if (i < m && j < n) {
-          const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
-          if (out_f64) {
-            double2 *C = reinterpret_cast<double2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
-            double2 v = make_double2((double)accR[a][b][r], (double)accI[a][b][r]);
-            if (ob.mode) {
-              const double2 o = C[idx];
-              v.x += o.x;
-              v.y += o.y;
-            }
-            C[idx] = v;
-          } else {
-            float2 *C = reinterpret_cast<float2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
-            float2 v = make_float2(accR[a][b][r], accI[a][b][r]);
-            if (ob.mode) {
-              const float2 o = C[idx];
-              v.x += o.x;
-              v.y += o.y;
-            }
-            C[idx] = v;
-          }
-        }
+  write_unit<float>(bufs, ob, t, part, i0, j0, wm, wn, lane, rowmode, accR, accI);
+}
+
+// ---------------------------------------------------------------------
+// Sum the partial tiles of split units in slot order and write C.
+template <typename CT>
+__global__ __launch_bounds__(256) void reduce_kernel(Bufs bufs, const tsc_outblock *__restrict__ blocks,
+                                                     const Reduce *__restrict__ reds, const void *part) {
+  const Reduce rd = reds[blockIdx.x];
+  const tsc_outblock ob = blocks[rd.blk];
+  const int i0 = rd.ti * TM, j0 = rd.tj * TN;
+  const bool out_f64 = (ob.c_buf == TSC_BUF_IN || ob.c_buf == TSC_BUF_OUT || sizeof(CT) == 8);
+  for (int e = threadIdx.x; e < TM * TN; e += 256) {
+    const int il = e % TM, jl = e / TM;
+    const int i = i0 + il, j = j0 + jl;
+    if (i >= ob.m || j >= ob.n) continue;
+    double sr = 0.0, si = 0.0;
+    for (int s = 0; s < rd.nslot; ++s) {
+      const int64_t pidx = (int64_t)(rd.slot0 + s) * (TM * TN) + e;
+      if (sizeof(CT) == 8) {
+        const double2 v = reinterpret_cast<const double2 *>(part)[pidx];
+        sr += v.x;
+        si += v.y;
+      } else {
+        const float2 v = reinterpret_cast<const float2 *>(part)[pidx];
+        sr += (double)v.x;
+        si += (double)v.y;
       }
+    }
+    const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
+    if (out_f64) {
+      double2 *C = reinterpret_cast<double2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+      double2 v = make_double2(sr, si);
+      if (ob.mode) {
+        const double2 o = C[idx];
+        v.x += o.x;
+        v.y += o.y;
+      }
+      C[idx] = v;
+    } else {
+      float2 *C = reinterpret_cast<float2 *>(const_cast<void *>(bufs.p[ob.c_buf]));
+      float2 v = make_float2((float)sr, (float)si);
+      if (ob.mode) {
+        const float2 o = C[idx];
+        v.x += o.x;
+        v.y += o.y;
+      }
+      C[idx] = v;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -435,7 +544,6 @@ __global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const ts
 __global__ void probe_kernel(float *out) {
   const int lane = threadIdx.x;
   const int i = lane & 15;
-  // f64 16x16x4
   {
     d4 c1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)i, 1.0, d4{0, 0, 0, 0}, 0, 0, 0);
     d4 c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(1.0, (double)i, d4{0, 0, 0, 0}, 0, 0, 0);
@@ -444,7 +552,6 @@ __global__ void probe_kernel(float *out) {
       out[0 * 512 + 256 + lane * 4 + r] = (float)c2[r];
     }
   }
-  // f32 16x16x4
   {
     f4 c1 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)i, 1.0f, f4{0, 0, 0, 0}, 0, 0, 0);
     f4 c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(1.0f, (float)i, f4{0, 0, 0, 0}, 0, 0, 0);
@@ -453,7 +560,6 @@ __global__ void probe_kernel(float *out) {
       out[1 * 512 + 256 + lane * 4 + r] = c2[r];
     }
   }
-  // bf16 16x16x32
   {
     bf8 vi, v1;
     for (int e = 0; e < 8; ++e) {
@@ -504,20 +610,29 @@ int probe_layouts(hipStream_t stream, int modes[3]) {
   return TSC_OK;
 }
 
-hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_blocks,
-                          const tsc_contrib *d_contribs, const Tile *d_tiles, int ntiles,
+hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_blocks, const tsc_contrib *d_contribs,
+                          const Tile *d_tiles, int ntiles, const Reduce *d_reds, int nreds, void *part,
                           hipStream_t stream, const int *modes) {
   if (ntiles <= 0) return hipSuccess;
   dim3 grid(ntiles), block(256);
   if (prec == TSC_PREC_F64) {
-    hipLaunchKernelGGL((grouped_kernel<double, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs,
-                       d_tiles, modes[0]);
+    hipLaunchKernelGGL((grouped_kernel<double, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
+                       modes[0], part);
   } else if (prec == TSC_PREC_F32) {
-    hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs,
-                       d_tiles, modes[1]);
+    hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
+                       modes[1], part);
   } else {
-    hipLaunchKernelGGL(grouped_kernel_bf16x3, grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
-                       modes[2]);
+    hipLaunchKernelGGL(grouped_kernel_bf16x3, grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles, modes[2],
+                       part);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nreds <= 0) return e;
+  if (prec == TSC_PREC_F64) {
+    hipLaunchKernelGGL(reduce_kernel<double>, dim3(nreds), dim3(256), 0, stream, bufs, d_blocks, d_reds,
+                       (const void *)part);
+  } else {
+    hipLaunchKernelGGL(reduce_kernel<float>, dim3(nreds), dim3(256), 0, stream, bufs, d_blocks, d_reds,
+                       (const void *)part);
   }
   return hipGetLastError();
 }

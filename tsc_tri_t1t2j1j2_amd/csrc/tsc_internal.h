@@ -34,6 +34,16 @@ void set_error(const std::string &msg);
 struct Tile {
   int32_t blk;
   int16_t ti, tj;
+  int32_t cbeg, cend;  // contribution range of this work unit
+  int32_t slot;        // -1: write C directly; else partial-tile slot
+  int32_t pad;
+};
+
+// Sum of the partial tiles [slot0, slot0 + nslot) into one output tile.
+struct Reduce {
+  int32_t blk;
+  int16_t ti, tj;
+  int32_t slot0, nslot;
 };
 
 // Base pointers of the four buffers a contribution may address.
@@ -45,7 +55,8 @@ struct Bufs {
 // prec: TSC_PREC_*; tiles are for the 64x64 tile shape.
 hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_blocks,
                           const tsc_contrib *d_contribs, const Tile *d_tiles, int ntiles,
-                          hipStream_t stream, const int *modes);
+                          const Reduce *d_reds, int nreds, void *part, hipStream_t stream,
+                          const int *modes);
 // Determine the accumulator row layout of the f64 / f32 / bf16 MFMAs.
 int probe_layouts(hipStream_t stream, int modes[3]);
 
@@ -95,6 +106,11 @@ struct tsc_plan {
   tsc_outblock *d_o[2] = {nullptr, nullptr};
   tsc::Tile *d_t[2] = {nullptr, nullptr};
   int nt[2] = {0, 0};
+  tsc::Reduce *d_r[2] = {nullptr, nullptr};
+  int nr[2] = {0, 0};
+  int64_t nslot[2] = {0, 0};           // partial-tile slots per stage
+  void *d_part64 = nullptr;            // partial tiles, FP64 path
+  void *d_part32 = nullptr;            // partial tiles, FP32 / BF16x3 paths
   double flops = 0.0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool timed = false;
