@@ -101,11 +101,16 @@ def test_dmrg_untruncated_equals_ed(ctx, Lx, Ny, Ne):
     assert abs(e - e_ed) < 1e-8, (e, e_ed)
 
 
-def test_dmrg_truncated_is_variational_and_close(ctx):
+def test_dmrg_truncated_is_variational_and_converges_in_M(ctx):
     Lx, Ny, Ne = 3, 4, 10
     b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
     e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
-    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=20, kept_max=60, sweeps=3, lanzero=1e-9, trun_err=1e-10)
-    e = d.run()
-    assert e >= e_ed - 1e-9
-    assert (e - e_ed) / (Lx * Ny) < 1e-4
+    es = []
+    for kmax in (30, 60, 200):
+        d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=min(20, kmax), kept_max=kmax, sweeps=3, lanzero=1e-10,
+                 trun_err=1e-12)
+        es.append(d.run())
+    assert all(e >= e_ed - 1e-9 for e in es), (es, e_ed)
+    assert es[0] >= es[1] - 1e-8 >= es[2] - 2e-8, es
+    assert (es[1] - e_ed) / (Lx * Ny) < 1e-3
+    assert (es[2] - e_ed) / (Lx * Ny) < 1e-7, (es[2], e_ed)

@@ -610,7 +610,7 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
     hipStreamSynchronize(ctx->stream);
     for (auto v : V) hipFree(v);
   };
-  std::vector<double> alpha(max_inner + 1, 0.0), beta(max_inner + 1, 0.0), z(max_inner + 1);
+  std::vector<double> alpha(max_inner + 1, 0.0), beta(max_inner + 1, 0.0), z(max_inner + 1), uu(max_inner + 1, 1.0);
   double lastmin = 1024.0;
   int matvecs = 0, iter_out = 0, last = 1;
   bool done = false;
@@ -636,6 +636,12 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
       const double2 *vv[1] = {V[0]};
       LZ_CALL(dots_re(ctx, vv, 1, V[0], n, &nn));
       LZ_CALL(scale(ctx, V[0], 1.0 / std::sqrt(nn), n));
+    }
+    {
+      double nn;
+      const double2 *vv[1] = {V[0]};
+      LZ_CALL(dots_re(ctx, vv, 1, V[0], n, &nn));
+      uu[0] = nn;
     }
     LZ_CALL(hpsi_dev(p, V[0], V[1], prec));
     ++matvecs;
@@ -667,18 +673,17 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
       // classical Gram-Schmidt with real overlaps, then normalise
       {
         std::vector<const double2 *> vv(V.begin(), V.begin() + last);
-        std::vector<double> uv(last), uu(last), cc(last);
+        std::vector<double> uv(last), cc(last);
         LZ_CALL(dots_re(ctx, vv.data(), last, V[last], n, uv.data()));
-        for (int j = 0; j < last; ++j) {
-          const double2 *one[1] = {V[j]};
-          LZ_CALL(dots_re(ctx, one, 1, V[j], n, &uu[j]));
-          cc[j] = -uv[j] / uu[j];
-        }
+        // <u_j|u_j> of the stored Krylov vectors (prod_uu of Gram_Schmidt_process)
+        for (int j = 0; j < last; ++j) cc[j] = -uv[j] / uu[j];
         LZ_CALL(combine(ctx, V[last], 1.0, vv.data(), cc.data(), last, n));
         double nn;
         const double2 *me[1] = {V[last]};
         LZ_CALL(dots_re(ctx, me, 1, V[last], n, &nn));
         LZ_CALL(scale(ctx, V[last], 1.0 / std::sqrt(nn), n));
+        LZ_CALL(dots_re(ctx, me, 1, V[last], n, &nn));
+        uu[last] = nn;
       }
       ++last;
       LZ_CALL(hpsi_dev(p, V[last - 1], V[last], prec));

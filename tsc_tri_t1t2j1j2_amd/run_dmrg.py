@@ -1,0 +1,62 @@
+"""Run the SU(2) DMRG for the t1-t2-J1-J2 t-J model on a YC cylinder on
+one GPU and print a JSON summary (energy per site, truncation error,
+timings). One line of progress per DMRG step goes to stderr.
+
+    python -m tsc_tri_t1t2j1j2_amd.run_dmrg --Lx 6 --Ny 4 --kept-max 256 \
+        --sweeps 6 --lanzero 1e-10 --prec f64
+"""
+import argparse
+import json
+import sys
+import time
+
+from . import capi
+from .dmrg import DMRG
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--Lx", type=int, default=6)
+    ap.add_argument("--Ny", type=int, default=4)
+    ap.add_argument("--t1", type=float, default=1.0)
+    ap.add_argument("--t2", type=float, default=0.2)
+    ap.add_argument("--J1", type=float, default=0.4)
+    ap.add_argument("--J2", type=float, default=None)
+    ap.add_argument("--delta", type=int, default=12)
+    ap.add_argument("--kept-min", type=int, default=110)
+    ap.add_argument("--kept-max", type=int, default=256)
+    ap.add_argument("--kept-maxf", type=int, default=None)
+    ap.add_argument("--sweeps", type=int, default=6)
+    ap.add_argument("--lanzero", type=float, default=1e-10)
+    ap.add_argument("--trun-err", type=float, default=1e-15)
+    ap.add_argument("--prec", default="f64", choices=["f64", "f32", "bf16x3"])
+    ap.add_argument("--refine", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=1234)
+    a = ap.parse_args(argv)
+    ctx = capi.Context(0)
+    t0 = time.perf_counter()
+
+    def log(msg):
+        print(f"[{time.perf_counter() - t0:8.1f}s] {msg}", file=sys.stderr, flush=True)
+
+    prec = capi.PREC_NAMES[a.prec]
+    d = DMRG(ctx, a.Lx, a.Ny, t1=a.t1, t2=a.t2, J1=a.J1, J2=a.J2, delta=a.delta, kept_min=a.kept_min,
+             kept_max=a.kept_max, kept_maxf=a.kept_maxf, sweeps=a.sweeps, lanzero=a.lanzero, trun_err=a.trun_err,
+             prec=prec, refine=a.refine if prec != capi.PREC_F64 else 0, seed=a.seed, log=log)
+    E = d.run()
+    wall = time.perf_counter() - t0
+    sweep_rows = [h for h in d.history if h[0].startswith("sweep")]
+    last = [h for h in sweep_rows if h[0] == f"sweep{a.sweeps}"]
+    out = {
+        "lattice": f"YC{a.Ny} {a.Ny}x{a.Lx}", "sites": d.N, "electrons": d.Ne, "kept_max": d.kept_maxf,
+        "precision": a.prec, "E": E, "E_per_site": E / d.N,
+        "E_min_last_sweep": min(h[2] for h in last) if last else E,
+        "max_trun_err_last_sweep": max(h[3] for h in last) if last else None,
+        "wall_s": round(wall, 2), "timers_s": {k: round(v, 2) for k, v in d.timers.items()},
+        "hpsi_flops_total": d.hpsi_flops, "steps": len(d.history),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
