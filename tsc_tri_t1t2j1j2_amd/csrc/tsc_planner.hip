@@ -1,0 +1,516 @@
+// Native SU(2) superblock planner: turns block descriptors and a term list
+// into the task arrays of the grouped contraction engine (host code).
+//
+// Replaces the per-call work of model_wave's kernels in
+// hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/get_sys.f90:1783-5204 (linear
+// quantum-number searches, 6j look-ups, per-pair GEMM calls): for every
+// term [X(L) x Y(R)]^0 it enumerates the sub-block transitions of the L side
+// (operator on the block or on the site of L = block (x) site) and of the R
+// side, combines them over the super basis with the 6j recoupling factor of
+// the cut, and emits one contribution per target window, sharing the
+// stage-1 intermediate X.psi between all targets that need it.
+//
+// Conventions (same as the host mirror in hamiltonian.py): quantum numbers
+// (N, 2S), site basis electron (1,1) then hole (0,0), enlarged basis order of
+// Get_basis (block.f90:2087-2189), super basis of Get_super_basis
+// (block.f90:2194-2245), c~ from c+ by <j'||c~||j> = (-1)^(j-j'+1/2)
+// conj(<j||c+||j'>), fermion sign (-1)^N of the part an odd operator passes.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+
+#include "tsc_internal.h"
+
+namespace {
+
+using tsc::set_error;
+
+// ---------------------------------------------------------------------
+// recoupling coefficients (Edmonds 7.1.5-7.1.8), doubled spins
+struct SixJCache {
+  std::unordered_map<uint64_t, double> m;
+  double operator()(int a, int b, int c, int d, int e, int f) {
+    const uint64_t k = ((uint64_t)a << 50) | ((uint64_t)b << 40) | ((uint64_t)c << 30) | ((uint64_t)d << 20) |
+                       ((uint64_t)e << 10) | (uint64_t)f;
+    auto it = m.find(k);
+    if (it != m.end()) return it->second;
+    const double v = tsc_w6j(a, b, c, d, e, f);
+    m.emplace(k, v);
+    return v;
+  }
+};
+
+inline double par(int x2) { return ((x2 / 2) % 2) ? -1.0 : 1.0; }
+
+struct Coef {
+  SixJCache w;
+  double embed_block(int ja2, int ja1, int js, int J2, int J1, int k2) {
+    const double v = w(ja2, J2, js, J1, ja1, k2);
+    return v == 0.0 ? 0.0 : par(ja2 + js + J1 + k2) * std::sqrt((double)(J1 + 1) * (J2 + 1)) * v;
+  }
+  double embed_site(int js2, int js1, int ja, int J2, int J1, int k2) {
+    const double v = w(js2, J2, ja, J1, js1, k2);
+    return v == 0.0 ? 0.0 : par(ja + js1 + J2 + k2) * std::sqrt((double)(J1 + 1) * (J2 + 1)) * v;
+  }
+  double scalar_pair(int ja2, int ja1, int js2, int js1, int J, int k2) {
+    const double v = w(ja2, ja1, k2, js1, js2, J);
+    return v == 0.0 ? 0.0 : par(ja1 + js2 + k2 + J) / std::sqrt((double)k2 + 1) * v;
+  }
+  double cut_pair(int jx2, int jx1, int jy2, int jy1, int S2, int k2) {
+    const double v = w(jx2, jx1, k2, jy1, jy2, S2);
+    return v == 0.0 ? 0.0 : par(jx1 + jy2 + k2 + S2) / std::sqrt((double)k2 + 1) * v;
+  }
+  static double ctilde_phase(int j_ket, int j_bra) { return par(j_ket - j_bra + 1); }
+};
+
+// site of the t-J model: electron, hole
+const int SITE_N[2] = {1, 0};
+const int SITE_J[2] = {1, 0};
+// reduced site operators: c+ (e<-h), S (e<-e), n (e<-e)
+struct SiteEl {
+  int bra, ket;
+  double v;
+};
+const SiteEl SITE_C = {0, 1, -1.4142135623730951};
+const SiteEl SITE_S = {0, 0, 1.2247448713915890};
+const SiteEl SITE_NUM = {0, 0, 1.4142135623730951};
+const int SITE_K2[3] = {1, 2, 0};
+const int SITE_DN[3] = {1, 0, 0};
+
+// ---------------------------------------------------------------------
+struct Sub {
+  int a, s, pos, d;
+};
+struct EnlSector {
+  int n, j, dim;
+  std::vector<Sub> subs;
+};
+struct Basis {
+  std::vector<EnlSector> sec;
+  std::map<std::pair<int, int>, std::vector<std::pair<int, int>>> where;  // (a,s) -> [(x,pos)]
+};
+
+Basis get_basis(const std::vector<tsc_sector> &blk) {
+  // pre-list, QuickSort_Integer (2S desc, then N desc), unique
+  std::vector<std::pair<int, int>> pre;
+  for (const auto &b : blk)
+    for (int s = 0; s < 2; ++s)
+      for (int j = std::abs(b.twos - SITE_J[s]); j <= b.twos + SITE_J[s]; j += 2) pre.push_back({b.n + SITE_N[s], j});
+  const size_t n = pre.size();
+  for (size_t i = 0; i < n; ++i)
+    for (size_t x = i + 1; x < n; ++x)
+      if (pre[i].second < pre[x].second) std::swap(pre[i], pre[x]);
+  for (size_t i = 0; i < n; ++i)
+    for (size_t x = i + 1; x < n; ++x)
+      if (pre[i].second == pre[x].second && pre[i].first < pre[x].first) std::swap(pre[i], pre[x]);
+  Basis B;
+  for (size_t i = 0; i < n; ++i) {
+    if (!B.sec.empty() && B.sec.back().n == pre[i].first && B.sec.back().j == pre[i].second) continue;
+    EnlSector e;
+    e.n = pre[i].first;
+    e.j = pre[i].second;
+    e.dim = 0;
+    for (int a = 0; a < (int)blk.size(); ++a)
+      for (int s = 0; s < 2; ++s)
+        if (blk[a].n + SITE_N[s] == e.n && std::abs(blk[a].twos - SITE_J[s]) <= e.j && e.j <= blk[a].twos + SITE_J[s]) {
+          e.subs.push_back(Sub{a, s, e.dim, blk[a].dim});
+          e.dim += blk[a].dim;
+        }
+    B.sec.push_back(e);
+  }
+  for (int x = 0; x < (int)B.sec.size(); ++x)
+    for (const auto &sb : B.sec[x].subs) B.where[{sb.a, sb.s}].push_back({x, sb.pos});
+  return B;
+}
+
+// matrix reference used by a transition: stored block at arena offset,
+// leading dimension, and the op code turning it into the L-side factor
+struct MatRef {
+  int64_t off;
+  int ld;
+  int op;  // TSC_OP_N or TSC_OP_C (c~ from c+)
+};
+
+struct Tr {
+  int x, p, d, x2, p2, d2;
+  double f;
+  int mat;  // index into mats, -1 = identity on the block part
+};
+
+struct Ctx {
+  const tsc_sb_desc *D;
+  Coef C;
+  std::vector<MatRef> mats;
+  std::map<std::tuple<int64_t, int>, int> mat_index;
+  int matref(int64_t off, int ld, int op) {
+    auto key = std::make_tuple(off, op);
+    auto it = mat_index.find(key);
+    if (it != mat_index.end()) return it->second;
+    mats.push_back(MatRef{off, ld, op});
+    mat_index[key] = (int)mats.size() - 1;
+    return (int)mats.size() - 1;
+  }
+};
+
+// operator entries (bra, ket, arena offset, ld, op, phase) of op or of its c~
+struct Ent {
+  int bra, ket;
+  int64_t off;
+  int ld, op;
+  double ph;
+};
+
+std::vector<Ent> entries(const tsc_op_desc &op, const std::vector<tsc_sector> &sec, bool tilde) {
+  std::vector<Ent> out;
+  for (int i = 0; i < op.nblk; ++i) {
+    const tsc_op_block &b = op.blk[i];
+    const int ld = std::max(1, sec[b.bra].dim);
+    if (!tilde)
+      out.push_back(Ent{b.bra, b.ket, b.off, ld, TSC_OP_N, 1.0});
+    else
+      out.push_back(Ent{b.ket, b.bra, b.off, ld, TSC_OP_C, Coef::ctilde_phase(sec[b.bra].twos, sec[b.ket].twos)});
+  }
+  return out;
+}
+
+// transitions of a block operator embedded in block (x) site
+void blk_transitions(Ctx &cx, const Basis &B, const std::vector<tsc_sector> &sec, const tsc_op_desc &op, bool tilde,
+                     std::vector<Tr> &out) {
+  for (const Ent &e : entries(op, sec, tilde)) {
+    const int ja2 = sec[e.bra].twos, ja = sec[e.ket].twos;
+    const int m = cx.matref(e.off, e.ld, e.op);
+    for (int s = 0; s < 2; ++s) {
+      auto it1 = B.where.find({e.ket, s});
+      auto it2 = B.where.find({e.bra, s});
+      if (it1 == B.where.end() || it2 == B.where.end()) continue;
+      for (auto [x, p] : it1->second)
+        for (auto [x2, p2] : it2->second) {
+          const double f = cx.C.embed_block(ja2, ja, SITE_J[s], B.sec[x2].j, B.sec[x].j, op.k2) * e.ph;
+          if (f != 0.0) out.push_back(Tr{x, p, sec[e.ket].dim, x2, p2, sec[e.bra].dim, f, m});
+        }
+    }
+  }
+}
+
+// transitions of a site operator (0 c+, 1 S, 2 n) embedded in block (x) site
+void site_transitions(Ctx &cx, const Basis &B, const std::vector<tsc_sector> &sec, int which, bool tilde,
+                      std::vector<Tr> &out) {
+  const SiteEl el = which == 0 ? SITE_C : (which == 1 ? SITE_S : SITE_NUM);
+  const int k2 = SITE_K2[which];
+  const bool odd = SITE_DN[which] % 2 != 0;
+  int s2 = el.bra, s1 = el.ket;
+  double v = el.v, ph = 1.0;
+  if (tilde) {
+    std::swap(s1, s2);
+    ph = Coef::ctilde_phase(SITE_J[el.bra], SITE_J[el.ket]);
+  }
+  for (int a = 0; a < (int)sec.size(); ++a) {
+    const double sgn = (odd && (sec[a].n % 2)) ? -1.0 : 1.0;
+    auto it1 = B.where.find({a, s1});
+    auto it2 = B.where.find({a, s2});
+    if (it1 == B.where.end() || it2 == B.where.end()) continue;
+    for (auto [x, p] : it1->second)
+      for (auto [x2, p2] : it2->second) {
+        const double f = cx.C.embed_site(SITE_J[s2], SITE_J[s1], sec[a].twos, B.sec[x2].j, B.sec[x].j, k2) * v * ph * sgn;
+        if (f != 0.0) out.push_back(Tr{x, p, sec[a].dim, x2, p2, sec[a].dim, f, -1});
+      }
+  }
+}
+
+// scalar side terms: block Hamiltonian and [T(block) x U(site)]^0 pairs
+void scalar_transitions(Ctx &cx, const Basis &B, const std::vector<tsc_sector> &sec, const tsc_op_desc *ham,
+                        const tsc_pair_desc *pairs, int npairs, const tsc_op_desc *ops, bool reduced,
+                        std::vector<Tr> &out) {
+  if (ham) {
+    for (int i = 0; i < ham->nblk; ++i) {
+      const tsc_op_block &b = ham->blk[i];
+      if (b.bra != b.ket) continue;
+      const int a = b.ket;
+      const int m = cx.matref(b.off, std::max(1, sec[a].dim), TSC_OP_N);
+      for (int x = 0; x < (int)B.sec.size(); ++x)
+        for (const Sub &sb : B.sec[x].subs)
+          if (sb.a == a) {
+            const double sc = reduced ? std::sqrt((double)B.sec[x].j + 1) : 1.0;
+            out.push_back(Tr{x, sb.pos, sb.d, x, sb.pos, sb.d, sc / std::sqrt((double)sec[a].twos + 1), m});
+          }
+    }
+  }
+  for (int q = 0; q < npairs; ++q) {
+    const tsc_pair_desc &pd = pairs[q];
+    const tsc_op_desc &T = ops[pd.block_op];
+    const SiteEl el = pd.site_op == 0 ? SITE_C : (pd.site_op == 1 ? SITE_S : SITE_NUM);
+    const bool uodd = SITE_DN[pd.site_op] % 2 != 0;
+    int us2 = el.bra, us1 = el.ket;
+    double uph = 1.0;
+    if (pd.site_tilde) {
+      std::swap(us1, us2);
+      uph = Coef::ctilde_phase(SITE_J[el.bra], SITE_J[el.ket]);
+    }
+    const std::vector<Ent> te = entries(T, sec, pd.block_tilde != 0);
+    for (int x = 0; x < (int)B.sec.size(); ++x) {
+      const int J = B.sec[x].j;
+      const double sc = reduced ? std::sqrt((double)J + 1) : 1.0;
+      for (const Sub &sb : B.sec[x].subs) {
+        if (sb.s != us1) continue;
+        const double sgn = (uodd && (sec[sb.a].n % 2)) ? -1.0 : 1.0;
+        for (const Ent &e : te) {
+          if (e.ket != sb.a) continue;
+          const Sub *tgt = nullptr;
+          for (const Sub &t2 : B.sec[x].subs)
+            if (t2.a == e.bra && t2.s == us2) tgt = &t2;
+          if (!tgt) continue;
+          double f = cx.C.scalar_pair(sec[e.bra].twos, sec[sb.a].twos, SITE_J[us2], SITE_J[us1], J, T.k2);
+          f *= pd.coef * el.v * e.ph * uph * sgn * sc;
+          if (f == 0.0) continue;
+          out.push_back(Tr{x, sb.pos, sb.d, x, tgt->pos, tgt->d, f, cx.matref(e.off, e.ld, e.op)});
+        }
+      }
+    }
+  }
+}
+
+void identity_transitions(const Basis &B, std::vector<Tr> &out) {
+  for (int x = 0; x < (int)B.sec.size(); ++x)
+    for (const Sub &sb : B.sec[x].subs) out.push_back(Tr{x, sb.pos, sb.d, x, sb.pos, sb.d, 1.0, -1});
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+struct tsc_tasks {
+  std::vector<tsc_contrib> c1, c2;
+  std::vector<tsc_outblock> o1, o2;
+  int64_t mid_len = 0;
+  int64_t psi_len = 0;
+  std::vector<int32_t> super_xy;   // (x, y, dx, dy) per psi sector
+  std::vector<int32_t> lsec, rsec; // (n, 2J, dim) of the enlarged bases
+};
+
+namespace {
+
+struct Builder {
+  tsc_tasks *T;
+  std::vector<std::vector<tsc_contrib>> per_out;
+  std::unordered_map<uint64_t, int64_t> mid;   // intermediate key -> MID offset
+
+  static tsc_contrib make(int64_t aoff, int lda, int abuf, int aop, int64_t boff, int ldb, int bbuf, int bop, int k,
+                          double f) {
+    tsc_contrib c;
+    std::memset(&c, 0, sizeof(c));
+    c.a_off = aoff;
+    c.lda = lda;
+    c.a_buf = (int8_t)abuf;
+    c.a_op = (int8_t)aop;
+    c.b_off = boff;
+    c.ldb = ldb;
+    c.b_buf = (int8_t)bbuf;
+    c.b_op = (int8_t)bop;
+    c.k = k;
+    c.coef_re = f;
+    return c;
+  }
+};
+
+}  // namespace
+
+extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
+  TSC_REQUIRE(D && out, "tsc_superblock_tasks arguments");
+  TSC_REQUIRE(D->nsys >= 0 && D->nenv >= 0 && D->nops >= 0 && D->nterms >= 0, "negative counts");
+  std::vector<tsc_sector> A(D->sys, D->sys + D->nsys), Bv(D->env, D->env + D->nenv);
+  for (int i = 0; i < D->nops; ++i) {
+    const tsc_op_desc &op = D->ops[i];
+    const std::vector<tsc_sector> &sec = op.side == 0 ? A : Bv;
+    for (int b = 0; b < op.nblk; ++b) {
+      const tsc_op_block &bl = op.blk[b];
+      TSC_REQUIRE(bl.bra >= 0 && bl.bra < (int)sec.size() && bl.ket >= 0 && bl.ket < (int)sec.size(),
+                  "operator block sector index");
+      TSC_REQUIRE(bl.off >= 0 && bl.off + (int64_t)sec[bl.bra].dim * sec[bl.ket].dim <= D->arena_len,
+                  "operator block outside the arena");
+    }
+  }
+  Ctx cx;
+  cx.D = D;
+  Basis L = get_basis(A), R = get_basis(Bv);
+  tsc_tasks *T = new tsc_tasks();
+  // super basis
+  std::vector<std::array<int, 4>> sb;
+  std::map<std::pair<int, int>, int> sbi;
+  std::vector<int64_t> soff;
+  int64_t off = 0;
+  for (int x = 0; x < (int)L.sec.size(); ++x)
+    for (int y = 0; y < (int)R.sec.size(); ++y)
+      if (R.sec[y].n == D->ntot - L.sec[x].n && std::abs(L.sec[x].j - R.sec[y].j) <= D->s2tot &&
+          D->s2tot <= L.sec[x].j + R.sec[y].j) {
+        sbi[{x, y}] = (int)sb.size();
+        sb.push_back({x, y, L.sec[x].dim, R.sec[y].dim});
+        soff.push_back(off);
+        off += (int64_t)L.sec[x].dim * R.sec[y].dim;
+      }
+  T->psi_len = off;
+  for (auto &e : sb) T->super_xy.insert(T->super_xy.end(), e.begin(), e.end());
+  for (auto &e : L.sec) T->lsec.insert(T->lsec.end(), {e.n, e.j, e.dim});
+  for (auto &e : R.sec) T->rsec.insert(T->rsec.end(), {e.n, e.j, e.dim});
+  // output windows
+  std::unordered_map<uint64_t, int> win;
+  auto wkey = [](int i, int p, int q) { return ((uint64_t)i << 44) | ((uint64_t)p << 22) | (uint64_t)q; };
+  for (int i = 0; i < (int)sb.size(); ++i) {
+    const int x = sb[i][0], y = sb[i][1], dx = sb[i][2];
+    for (const Sub &ls : L.sec[x].subs)
+      for (const Sub &rs : R.sec[y].subs) {
+        tsc_outblock ob;
+        std::memset(&ob, 0, sizeof(ob));
+        ob.c_off = soff[i] + ls.pos + (int64_t)rs.pos * dx;
+        ob.ldc = std::max(1, dx);
+        ob.m = ls.d;
+        ob.n = rs.d;
+        ob.c_buf = TSC_BUF_OUT;
+        win[wkey(i, ls.pos, rs.pos)] = (int)T->o2.size();
+        T->o2.push_back(ob);
+      }
+  }
+  Builder bd;
+  bd.T = T;
+  bd.per_out.resize(T->o2.size());
+
+  auto cross = [&](const std::vector<Tr> &TL, const std::vector<Tr> &TR, double g, bool cut, int k2, bool y_odd,
+                   int lterm) {
+    std::vector<std::vector<const Tr *>> byx(L.sec.size()), byy(R.sec.size());
+    for (const Tr &t : TL) byx[t.x].push_back(&t);
+    for (const Tr &t : TR) byy[t.x].push_back(&t);
+    for (int i = 0; i < (int)sb.size(); ++i) {
+      const int x = sb[i][0], y = sb[i][1], dx = sb[i][2];
+      if (byx[x].empty() || byy[y].empty()) continue;
+      const int Jx = L.sec[x].j, Jy = R.sec[y].j;
+      const double fsgn = (y_odd && (L.sec[x].n % 2)) ? -1.0 : 1.0;
+      for (const Tr *tl : byx[x])
+        for (const Tr *tr : byy[y]) {
+          auto it = sbi.find({tl->x2, tr->x2});
+          if (it == sbi.end()) continue;
+          const int i2 = it->second;
+          double f = g * tl->f * tr->f;
+          if (cut) f *= cx.C.cut_pair(L.sec[tl->x2].j, Jx, R.sec[tr->x2].j, Jy, D->s2tot, k2) * fsgn;
+          if (f == 0.0) continue;
+          const int w = win[wkey(i2, tl->p2, tr->p2)];
+          const int64_t src = soff[i] + tl->p + (int64_t)tr->p * dx;
+          if (tl->mat >= 0 && tr->mat >= 0) {
+            const MatRef &lm = cx.mats[tl->mat];
+            const MatRef &rm = cx.mats[tr->mat];
+            const uint64_t key = ((uint64_t)tl->mat << 40) ^ ((uint64_t)i << 24) ^ ((uint64_t)tl->p << 12) ^
+                                 (uint64_t)tr->p ^ ((uint64_t)lterm << 58);
+            // exact key check through a composite map (collisions resolved below)
+            int64_t moff;
+            auto mit = bd.mid.find(key);
+            if (mit != bd.mid.end()) {
+              moff = mit->second;
+            } else {
+              moff = T->mid_len;
+              T->mid_len += (int64_t)tl->d2 * tr->d;
+              bd.mid.emplace(key, moff);
+              tsc_outblock ob;
+              std::memset(&ob, 0, sizeof(ob));
+              ob.c_off = moff;
+              ob.ldc = std::max(1, tl->d2);
+              ob.m = tl->d2;
+              ob.n = tr->d;
+              ob.c_buf = TSC_BUF_MID;
+              ob.cbeg = (int32_t)T->c1.size();
+              ob.cend = ob.cbeg + 1;
+              T->o1.push_back(ob);
+              T->c1.push_back(Builder::make(lm.off, lm.ld, TSC_BUF_OPS, lm.op, src, std::max(1, dx), TSC_BUF_IN,
+                                            TSC_OP_N, tl->d, 1.0));
+            }
+            bd.per_out[w].push_back(Builder::make(moff, std::max(1, tl->d2), TSC_BUF_MID, TSC_OP_N, rm.off, rm.ld,
+                                                  TSC_BUF_OPS, rm.op == TSC_OP_N ? TSC_OP_T : TSC_OP_CONJ, tr->d, f));
+          } else if (tl->mat >= 0) {
+            const MatRef &lm = cx.mats[tl->mat];
+            bd.per_out[w].push_back(Builder::make(lm.off, lm.ld, TSC_BUF_OPS, lm.op, src, std::max(1, dx), TSC_BUF_IN,
+                                                  TSC_OP_N, tl->d, f));
+          } else if (tr->mat >= 0) {
+            const MatRef &rm = cx.mats[tr->mat];
+            bd.per_out[w].push_back(Builder::make(src, std::max(1, dx), TSC_BUF_IN, TSC_OP_N, rm.off, rm.ld,
+                                                  TSC_BUF_OPS, rm.op == TSC_OP_N ? TSC_OP_T : TSC_OP_CONJ, tr->d, f));
+          } else {
+            bd.per_out[w].push_back(Builder::make(src, std::max(1, dx), TSC_BUF_IN, TSC_OP_N, 0, 1, 0, TSC_OP_I, 0, f));
+          }
+        }
+    }
+  };
+
+  std::vector<Tr> idL, idR, sL, sR;
+  identity_transitions(L, idL);
+  identity_transitions(R, idR);
+  scalar_transitions(cx, L, A, D->sys_ham, D->lpairs, D->nlpairs, D->ops, false, sL);
+  scalar_transitions(cx, R, Bv, D->env_ham, D->rpairs, D->nrpairs, D->ops, false, sR);
+  cross(sL, idR, 1.0, false, 0, false, 0);
+  cross(idL, sR, 1.0, false, 0, false, 0);
+  for (int t = 0; t < D->nterms; ++t) {
+    const tsc_term_desc &td = D->terms[t];
+    std::vector<Tr> TL, TR;
+    int k2 = 0;
+    bool yodd = false;
+    if (td.l_kind == 1) {
+      blk_transitions(cx, L, A, D->ops[td.l_op], td.l_tilde != 0, TL);
+      k2 = D->ops[td.l_op].k2;
+    } else {
+      site_transitions(cx, L, A, td.l_op, td.l_tilde != 0, TL);
+      k2 = SITE_K2[td.l_op];
+    }
+    if (td.r_kind == 1) {
+      blk_transitions(cx, R, Bv, D->ops[td.r_op], td.r_tilde != 0, TR);
+      yodd = D->ops[td.r_op].up_dif % 2 != 0;
+    } else {
+      site_transitions(cx, R, Bv, td.r_op, td.r_tilde != 0, TR);
+      yodd = SITE_DN[td.r_op] % 2 != 0;
+    }
+    cross(TL, TR, td.coef, true, k2, yodd, t + 1);
+  }
+  for (size_t w = 0; w < T->o2.size(); ++w) {
+    T->o2[w].cbeg = (int32_t)T->c2.size();
+    T->c2.insert(T->c2.end(), bd.per_out[w].begin(), bd.per_out[w].end());
+    T->o2[w].cend = (int32_t)T->c2.size();
+  }
+  *out = T;
+  return TSC_OK;
+}
+
+extern "C" int tsc_tasks_info(const tsc_tasks *T, int64_t *counts) {
+  TSC_REQUIRE(T && counts, "tsc_tasks_info arguments");
+  counts[0] = (int64_t)T->c1.size();
+  counts[1] = (int64_t)T->o1.size();
+  counts[2] = (int64_t)T->c2.size();
+  counts[3] = (int64_t)T->o2.size();
+  counts[4] = T->mid_len;
+  counts[5] = T->psi_len;
+  counts[6] = (int64_t)T->super_xy.size() / 4;
+  counts[7] = (int64_t)T->lsec.size() / 3;
+  counts[8] = (int64_t)T->rsec.size() / 3;
+  return TSC_OK;
+}
+
+extern "C" int tsc_tasks_copy(const tsc_tasks *T, tsc_contrib *c1, tsc_outblock *o1, tsc_contrib *c2,
+                              tsc_outblock *o2, int32_t *super_xy, int32_t *lsec, int32_t *rsec) {
+  TSC_REQUIRE(T, "tsc_tasks_copy: null tasks");
+  if (c1) std::copy(T->c1.begin(), T->c1.end(), c1);
+  if (o1) std::copy(T->o1.begin(), T->o1.end(), o1);
+  if (c2) std::copy(T->c2.begin(), T->c2.end(), c2);
+  if (o2) std::copy(T->o2.begin(), T->o2.end(), o2);
+  if (super_xy) std::copy(T->super_xy.begin(), T->super_xy.end(), super_xy);
+  if (lsec) std::copy(T->lsec.begin(), T->lsec.end(), lsec);
+  if (rsec) std::copy(T->rsec.begin(), T->rsec.end(), rsec);
+  return TSC_OK;
+}
+
+extern "C" int tsc_tasks_plan(tsc_ctx *ctx, const tsc_tasks *T, const double *arena, int64_t arena_len,
+                              tsc_plan **plan) {
+  TSC_REQUIRE(ctx && T && plan, "tsc_tasks_plan arguments");
+  return tsc::plan_create_impl(ctx, arena, nullptr, arena_len, T->mid_len, T->c1.data(), (int32_t)T->c1.size(),
+                               T->o1.data(), (int32_t)T->o1.size(), T->c2.data(), (int32_t)T->c2.size(),
+                               T->o2.data(), (int32_t)T->o2.size(), T->psi_len, T->psi_len, plan);
+}
+
+extern "C" int tsc_tasks_destroy(tsc_tasks *T) {
+  delete T;
+  return TSC_OK;
+}
