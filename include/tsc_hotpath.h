@@ -226,6 +226,72 @@ int tsc_update_trun(tsc_ctx *ctx, int32_t nblk, const int32_t *row, const int32_
                     const double *ori_c128, int64_t ori_elems, const double *u_c128,
                     int64_t u_elems, double *eff_c128, int64_t eff_elems, int precision);
 
+/*
+ * ---- native SU(2) superblock planner (host) -----------------------------
+ * Builds the grouped-contraction task arrays of model_wave for one
+ * superblock [sys block, sys site | env block, env site] from block
+ * descriptors, replacing the sector scans and quantum-number searches of
+ * model_wave and its kernels (get_sys.f90:1783-5204, 6915-7393). Operator
+ * matrices (reduced elements, bra x ket, column-major) live in one host
+ * arena; blocks refer to them by offset. Sectors are (N, 2S, dim).
+ * Term kinds: 1 = block operator ops[op], 2 = site operator op (0 c+,
+ * 1 S, 2 n); tilde = use the annihilation tensor c~ derived from c+.
+ * Every bond product is one term [X(sys side) x Y(env side)]^0 with coef;
+ * block-site products inside one side are pairs (scalar on that side).
+ */
+typedef struct tsc_sector {
+  int32_t n, twos, dim;
+} tsc_sector;
+
+typedef struct tsc_op_block {
+  int32_t bra, ket; /* sector indices of the block the operator acts on */
+  int64_t off;      /* arena offset of the bra_dim x ket_dim matrix     */
+} tsc_op_block;
+
+typedef struct tsc_op_desc {
+  int32_t side; /* 0 sys block, 1 env block */
+  int32_t up_dif, k2, nblk;
+  const tsc_op_block *blk;
+} tsc_op_desc;
+
+typedef struct tsc_pair_desc {
+  double coef;
+  int32_t block_op, block_tilde, site_op, site_tilde;
+} tsc_pair_desc;
+
+typedef struct tsc_term_desc {
+  double coef;
+  int32_t l_kind, l_op, l_tilde, r_kind, r_op, r_tilde;
+} tsc_term_desc;
+
+typedef struct tsc_sb_desc {
+  int32_t nsys, nenv;
+  const tsc_sector *sys, *env;
+  int64_t arena_len;
+  int32_t nops;
+  const tsc_op_desc *ops;
+  const tsc_op_desc *sys_ham, *env_ham;
+  int32_t nlpairs;
+  const tsc_pair_desc *lpairs;
+  int32_t nrpairs;
+  const tsc_pair_desc *rpairs;
+  int32_t nterms;
+  const tsc_term_desc *terms;
+  int32_t ntot, s2tot;
+} tsc_sb_desc;
+
+typedef struct tsc_tasks tsc_tasks;
+int tsc_superblock_tasks(const tsc_sb_desc *desc, tsc_tasks **out);
+/* counts: nc1, no1, nc2, no2, mid_len, psi_len, n_super, n_lsec, n_rsec */
+int tsc_tasks_info(const tsc_tasks *t, int64_t *counts9);
+/* any output pointer may be NULL; super_xy: (x, y, dx, dy) per psi sector,
+ * lsec/rsec: (N, 2J, dim) of the enlarged sys/env bases */
+int tsc_tasks_copy(const tsc_tasks *t, tsc_contrib *c1, tsc_outblock *o1, tsc_contrib *c2,
+                   tsc_outblock *o2, int32_t *super_xy, int32_t *lsec, int32_t *rsec);
+int tsc_tasks_plan(tsc_ctx *ctx, const tsc_tasks *t, const double *arena_c128, int64_t arena_len,
+                   tsc_plan **plan);
+int tsc_tasks_destroy(tsc_tasks *t);
+
 /* Racah 6j symbol with doubled arguments (W6JS, block.f90:3744-3795) */
 double tsc_w6j(int32_t j1, int32_t j2, int32_t j3, int32_t l1, int32_t l2, int32_t l3);
 

@@ -16,7 +16,7 @@ def test_driver_on_fake_device(monkeypatch, kmax):
     e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
     log = []
     d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=min(20, kmax), kept_max=kmax, sweeps=2, lanzero=1e-10, trun_err=1e-12,
-             log=log.append)
+             log=log.append, native_planner=False)
     e = d.run()
     print("\n".join(log))
     assert e >= e_ed - 1e-9

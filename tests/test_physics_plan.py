@@ -6,8 +6,10 @@ import pytest
 
 from oracle import dmrg_exact, su2_dense, tj_model
 from tests.model_convert import block_to_product, oracle_superblock
+from oracle import plan_exec
 from tests.plan_eval import apply_numpy
 from tsc_tri_t1t2j1j2_amd import hamiltonian as ham
+from tsc_tri_t1t2j1j2_amd import native
 
 CASES = [(2, 3, 4, 0, 2), (2, 3, 5, 1, 2), (2, 4, 6, 0, 3), (2, 4, 6, 0, 2), (3, 3, 8, 0, 3), (2, 4, 7, 1, 1),
          (3, 4, 10, 0, 4)]
@@ -80,3 +82,16 @@ def test_renormalize_plan_matches_oracle():
     for i in keep:
         for name in ("c", "S", "n"):
             check(got.ops[i][name], want.ops[i][name])
+
+
+@pytest.mark.parametrize("Lx,Ny,Ne,S2,l", CASES)
+def test_native_planner_matches_oracle(Lx, Ny, Ne, S2, l):
+    osb, psb = _setup(Lx, Ny, Ne, S2, l)
+    nt = native.NativeTasks(psb)
+    assert nt.psi_len == osb.n
+    c1, o1, c2, o2 = nt.arrays()
+    rng = np.random.default_rng(Lx * 1000 + Ny * 10 + l)
+    v = rng.normal(size=osb.n) + 1j * rng.normal(size=osb.n)
+    got = plan_exec.execute(nt.arena, c1, o1, c2, o2, nt.mid_len, v, osb.n)
+    want = osb.matvec(v)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)

@@ -40,7 +40,8 @@ EXPORTS = [
     "tsc_plan_create", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
     "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_graph", "tsc_plan_last_times",
     "tsc_lanczos", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate",
-    "tsc_update_trun", "tsc_w6j",
+    "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_copy",
+    "tsc_tasks_plan", "tsc_tasks_destroy",
 ]
 
 

@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <array>
+#include <tuple>
 #include <unordered_map>
 
 #include "tsc_internal.h"
@@ -290,10 +292,17 @@ struct tsc_tasks {
 
 namespace {
 
+struct MidKey {
+  int mat, i, p, q;
+  bool operator<(const MidKey &o) const {
+    return mat != o.mat ? mat < o.mat : (i != o.i ? i < o.i : (p != o.p ? p < o.p : q < o.q));
+  }
+};
+
 struct Builder {
   tsc_tasks *T;
   std::vector<std::vector<tsc_contrib>> per_out;
-  std::unordered_map<uint64_t, int64_t> mid;   // intermediate key -> MID offset
+  std::map<MidKey, int64_t> mid;   // (L matrix, psi sector, row pos, col pos) -> MID offset
 
   static tsc_contrib make(int64_t aoff, int lda, int abuf, int aop, int64_t boff, int ldb, int bbuf, int bop, int k,
                           double f) {
@@ -353,8 +362,15 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
   for (auto &e : L.sec) T->lsec.insert(T->lsec.end(), {e.n, e.j, e.dim});
   for (auto &e : R.sec) T->rsec.insert(T->rsec.end(), {e.n, e.j, e.dim});
   // output windows
+  // window of (psi sector i, row offset p, column offset q); p, q < 2^24, i < 2^16 checked below
   std::unordered_map<uint64_t, int> win;
-  auto wkey = [](int i, int p, int q) { return ((uint64_t)i << 44) | ((uint64_t)p << 22) | (uint64_t)q; };
+  auto wkey = [](int i, int p, int q) { return ((uint64_t)i << 48) | ((uint64_t)p << 24) | (uint64_t)q; };
+  for (const auto &e : sb)
+    if (e[2] >= (1 << 24) || e[3] >= (1 << 24) || sb.size() >= (1u << 16)) {
+      delete T;
+      set_error("superblock too large for the window index");
+      return TSC_EINVAL;
+    }
   for (int i = 0; i < (int)sb.size(); ++i) {
     const int x = sb[i][0], y = sb[i][1], dx = sb[i][2];
     for (const Sub &ls : L.sec[x].subs)
@@ -374,8 +390,7 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
   bd.T = T;
   bd.per_out.resize(T->o2.size());
 
-  auto cross = [&](const std::vector<Tr> &TL, const std::vector<Tr> &TR, double g, bool cut, int k2, bool y_odd,
-                   int lterm) {
+  auto cross = [&](const std::vector<Tr> &TL, const std::vector<Tr> &TR, double g, bool cut, int k2, bool y_odd) {
     std::vector<std::vector<const Tr *>> byx(L.sec.size()), byy(R.sec.size());
     for (const Tr &t : TL) byx[t.x].push_back(&t);
     for (const Tr &t : TR) byy[t.x].push_back(&t);
@@ -397,9 +412,7 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
           if (tl->mat >= 0 && tr->mat >= 0) {
             const MatRef &lm = cx.mats[tl->mat];
             const MatRef &rm = cx.mats[tr->mat];
-            const uint64_t key = ((uint64_t)tl->mat << 40) ^ ((uint64_t)i << 24) ^ ((uint64_t)tl->p << 12) ^
-                                 (uint64_t)tr->p ^ ((uint64_t)lterm << 58);
-            // exact key check through a composite map (collisions resolved below)
+            const MidKey key{tl->mat, i, tl->p, tr->p};
             int64_t moff;
             auto mit = bd.mid.find(key);
             if (mit != bd.mid.end()) {
@@ -443,8 +456,8 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
   identity_transitions(R, idR);
   scalar_transitions(cx, L, A, D->sys_ham, D->lpairs, D->nlpairs, D->ops, false, sL);
   scalar_transitions(cx, R, Bv, D->env_ham, D->rpairs, D->nrpairs, D->ops, false, sR);
-  cross(sL, idR, 1.0, false, 0, false, 0);
-  cross(idL, sR, 1.0, false, 0, false, 0);
+  cross(sL, idR, 1.0, false, 0, false);
+  cross(idL, sR, 1.0, false, 0, false);
   for (int t = 0; t < D->nterms; ++t) {
     const tsc_term_desc &td = D->terms[t];
     std::vector<Tr> TL, TR;
@@ -464,7 +477,7 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
       site_transitions(cx, R, Bv, td.r_op, td.r_tilde != 0, TR);
       yodd = SITE_DN[td.r_op] % 2 != 0;
     }
-    cross(TL, TR, td.coef, true, k2, yodd, t + 1);
+    cross(TL, TR, td.coef, true, k2, yodd);
   }
   for (size_t w = 0; w < T->o2.size(); ++w) {
     T->o2[w].cbeg = (int32_t)T->c2.size();

@@ -18,6 +18,7 @@ import numpy as np
 from . import capi
 from . import hamiltonian as ham
 from . import lattice
+from . import native
 from .blocks import SITE_SECTORS, TotalBlock, TotalModel, site_operators
 
 
@@ -42,7 +43,7 @@ def start_vector(super_basis, seed):
 class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
-                 prec=capi.PREC_F64, refine=0, seed=1234, log=None):
+                 prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -57,6 +58,7 @@ class DMRG:
         self.trun_err = trun_err
         self.prec, self.refine = prec, refine
         self.seed = seed
+        self.native_planner = native_planner
         self.bonds = lattice.bonds(Lx, Ny, t1, t2, J1, self.J2)
         self.nb = {}
         for (a, b, t, J) in self.bonds:
@@ -115,8 +117,12 @@ class DMRG:
         the grown block(s)."""
         t0 = time.perf_counter()
         sb = ham.Superblock(sys, env, s1, s2, bonds, ntot, self.S2)
-        pb = sb.plan_builder()
-        plan = pb.build(self.ctx, sb.super.dim, sb.super.dim)
+        if self.native_planner:
+            tasks = native.NativeTasks(sb)
+            plan = tasks.plan(self.ctx)
+            tasks.close()
+        else:
+            plan = sb.plan_builder().build(self.ctx, sb.super.dim, sb.super.dim)
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         psi = self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
