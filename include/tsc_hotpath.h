@@ -100,9 +100,11 @@ int tsc_ctx_create(int device, tsc_ctx **out);
 int tsc_ctx_destroy(tsc_ctx *ctx);
 int tsc_ctx_sync(tsc_ctx *ctx);
 /*
- * Accumulator row layout of the f64 / f32 / bf16 16x16 MFMAs found by the
- * probe run at context creation (0: row = 4*(lane/16)+r, 1: row =
- * lane/16+4r). Diagnostic; has no counterpart in the reference.
+ * MFMA accumulator layouts found by the probe run at context creation:
+ * modes3[0], modes3[1] = row layout of the f64 / f32 16x16 MFMAs (0: row =
+ * 4*(lane/16)+r, 1: row = lane/16+4r); modes3[2] = 0 if the 32x32x16 bf16
+ * layout is row = 8*(r/4)+4*(lane/32)+r%4 (BF16x3 usable), -1 otherwise.
+ * Diagnostic; has no counterpart in the reference.
  */
 int tsc_ctx_mfma_layout(const tsc_ctx *ctx, int32_t *modes3);
 

@@ -104,6 +104,135 @@ def test_grouped_contraction_all_op_modes(ctx, prec):
         assert _rel(got, acc) < TOL[prec] * 10, (prec, m, n)
 
 
+def _two_stage_case(rng, nout=8):
+    """Random two-stage problem: stage 1 builds intermediates as sums of
+    arena x psi products (long sums, so work units are split), stage 2 reads
+    each intermediate as a row operand or as a transposed operand (some in
+    both roles) with random ops and complex coefficients."""
+    ops, vin = [], []
+    nops = nin = 0
+    c1, o1, c2, o2 = [], [], [], []
+    mids = []  # (offset, ld, M)
+    mid_off = 0
+
+    def rnd(r, c):
+        return rng.normal(size=(r, c)) + 1j * rng.normal(size=(r, c))
+
+    def arena(M):
+        nonlocal nops
+        ops.append(M.ravel(order="F"))
+        nops += M.size
+        return nops - M.size
+
+    def psi(M):
+        nonlocal nin
+        vin.append(M.ravel(order="F"))
+        nin += M.size
+        return nin - M.size
+
+    def new_mid(mb, nb):
+        nonlocal mid_off
+        ld = mb + int(rng.integers(0, 2))
+        ob = np.zeros((), capi.OUTBLOCK_DTYPE)
+        ob["c_off"], ob["ldc"], ob["m"], ob["n"], ob["c_buf"] = mid_off, ld, mb, nb, capi.BUF_MID
+        ob["cbeg"] = len(c1)
+        acc = np.zeros((mb, nb), np.complex128)
+        for t in range(int(rng.integers(1, 7))):
+            e = np.zeros((), capi.CONTRIB_DTYPE)
+            coef = complex(rng.normal(), rng.normal()) if t % 3 else 1.0
+            e["coef_re"], e["coef_im"] = coef.real, coef.imag
+            if t == 5:  # elementwise psi term
+                X = rnd(mb, nb)
+                e["a_off"], e["lda"], e["a_buf"], e["a_op"], e["b_op"] = psi(X), mb, capi.BUF_IN, capi.OP_N, capi.OP_I
+                acc += coef * X
+            else:
+                k = int(rng.integers(1, 90))
+                aop, bop = int(rng.integers(0, 4)), int(rng.integers(0, 4))
+                A = rnd(*((mb, k) if aop in (capi.OP_N, capi.OP_CONJ) else (k, mb)))
+                B = rnd(*((k, nb) if bop in (capi.OP_N, capi.OP_CONJ) else (nb, k)))
+                e["a_off"], e["lda"], e["a_buf"], e["a_op"] = arena(A), A.shape[0], capi.BUF_OPS, aop
+                e["b_off"], e["ldb"], e["b_buf"], e["b_op"] = psi(B), B.shape[0], capi.BUF_IN, bop
+                e["k"] = k
+                acc += coef * (_op(A, aop) @ _op(B, bop))
+            c1.append(e)
+        ob["cend"] = len(c1)
+        o1.append(ob)
+        mids.append((mid_off, ld, acc))
+        mid_off += ld * nb
+        return len(mids) - 1
+
+    expected, out_off = [], 0
+    for b in range(nout):
+        m, n = (int(x) for x in rng.integers(1, 150, 2))
+        if b > 0 and rng.random() < 0.5:
+            # reuse an intermediate in the other role: its dims fix (m or n)
+            idx = int(rng.integers(0, len(mids)))
+            mb, nb = mids[idx][2].shape
+            m = nb if rng.random() < 0.5 else mb
+        ob = np.zeros((), capi.OUTBLOCK_DTYPE)
+        ob["c_off"], ob["ldc"], ob["m"], ob["n"], ob["c_buf"] = out_off, m, m, n, capi.BUF_OUT
+        ob["cbeg"] = len(c2)
+        acc = np.zeros((m, n), np.complex128)
+        for t in range(int(rng.integers(1, 6))):
+            e = np.zeros((), capi.CONTRIB_DTYPE)
+            coef = complex(rng.choice([0.5, -1.25, 2.0]), 0.0) if t % 2 else complex(rng.normal(), rng.normal())
+            e["coef_re"], e["coef_im"] = coef.real, coef.imag
+            op = int(rng.integers(0, 4))
+            # find or make an intermediate usable as A (rows m) or as B (columns n)
+            cand = [(i, r) for i, (_, _, M) in enumerate(mids) for r in ("A", "B")
+                    if (r == "A" and (M.shape[0] if op in (capi.OP_N, capi.OP_CONJ) else M.shape[1]) == m)
+                    or (r == "B" and (M.shape[1] if op in (capi.OP_N, capi.OP_CONJ) else M.shape[0]) == n)]
+            if cand and rng.random() < 0.6:
+                idx, role = cand[int(rng.integers(0, len(cand)))]
+            else:
+                role = "A" if rng.random() < 0.5 else "B"
+                k = int(rng.integers(1, 120))
+                if role == "A":
+                    dims = (m, k) if op in (capi.OP_N, capi.OP_CONJ) else (k, m)
+                else:
+                    dims = (k, n) if op in (capi.OP_N, capi.OP_CONJ) else (n, k)
+                idx = new_mid(*dims)
+            moff, mld, M = mids[idx]
+            Mop = _op(M, op)
+            if role == "A":
+                k = Mop.shape[1]
+                oop = int(rng.integers(0, 4))
+                X = rnd(*((k, n) if oop in (capi.OP_N, capi.OP_CONJ) else (n, k)))
+                e["a_off"], e["lda"], e["a_buf"], e["a_op"] = moff, mld, capi.BUF_MID, op
+                e["b_off"], e["ldb"], e["b_buf"], e["b_op"] = arena(X), X.shape[0], capi.BUF_OPS, oop
+                acc += coef * (Mop @ _op(X, oop))
+            else:
+                k = Mop.shape[0]
+                oop = int(rng.integers(0, 4))
+                X = rnd(*((m, k) if oop in (capi.OP_N, capi.OP_CONJ) else (k, m)))
+                e["a_off"], e["lda"], e["a_buf"], e["a_op"] = arena(X), X.shape[0], capi.BUF_OPS, oop
+                e["b_off"], e["ldb"], e["b_buf"], e["b_op"] = moff, mld, capi.BUF_MID, op
+                acc += coef * (_op(X, oop) @ Mop)
+            e["k"] = k
+            c2.append(e)
+        ob["cend"] = len(c2)
+        o2.append(ob)
+        expected.append((out_off, m, acc))
+        out_off += m * n
+    return (np.concatenate(ops), nops, np.concatenate(vin), nin, mid_off, c1, o1, c2, o2, out_off, expected)
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_F64, capi.PREC_F32, capi.PREC_BF16X3])
+def test_two_stage_intermediates_in_both_roles(ctx, prec):
+    rng = np.random.default_rng(500 + prec)
+    ops, nops, vin, nin, nmid, c1, o1, c2, o2, nout, expected = _two_stage_case(rng)
+    plan = capi.Plan(ctx, ops, nops, nmid, np.array(c1, capi.CONTRIB_DTYPE), np.array(o1, capi.OUTBLOCK_DTYPE),
+                     np.array(c2, capi.CONTRIB_DTYPE), np.array(o2, capi.OUTBLOCK_DTYPE), nin, nout)
+    out = plan.apply_host(vin, prec)
+    for off, ldc, acc in expected:
+        m, n = acc.shape
+        got = out[off:off + ldc * n].reshape((ldc, n), order="F")[:m, :]
+        assert _rel(got, acc) < TOL[prec] * 10, (prec, m, n)
+    # a second call on the same plan (psi panels repacked, intermediates rewritten)
+    out2 = plan.apply_host(2.0 * vin, prec)
+    assert _rel(out2, 2.0 * out) < TOL[prec] * 10
+
+
 def test_plan_rejects_out_of_bounds_operand(ctx):
     e = np.zeros(1, capi.CONTRIB_DTYPE)
     e["a_off"], e["lda"], e["a_buf"], e["a_op"] = 0, 4, capi.BUF_OPS, capi.OP_N

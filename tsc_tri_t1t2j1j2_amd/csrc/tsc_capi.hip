@@ -243,16 +243,13 @@ int check_stage(const tsc_contrib *c, int32_t nc, const tsc_outblock *o, int32_t
   return TSC_OK;
 }
 
+}  // namespace
+
+namespace tsc {
 // Work units of one stage: every 64x64 tile of every output block; tiles
 // whose contribution list is long are cut into contiguous chunks of about
 // `target` k-steps that write partial tiles (summed by a Reduce entry).
 // Units are ordered by work, largest first.
-struct Units {
-  std::vector<Tile> tiles;
-  std::vector<Reduce> reds;
-  int64_t nslot = 0;
-};
-
 Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no) {
   auto ccost = [&](int32_t ci) {
     return (c[ci].a_op == TSC_OP_I || c[ci].b_op == TSC_OP_I) ? 1.0 : (double)std::max(c[ci].k, 1);
@@ -310,6 +307,9 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no) {
   u.tiles.swap(sorted);
   return u;
 }
+}  // namespace tsc
+
+namespace {
 
 template <typename T>
 int upload(T **dst, const T *src, size_t n, hipStream_t s) {
@@ -340,6 +340,7 @@ extern "C" int tsc_plan_destroy(tsc_plan *p) {
   }
   hipFree(p->d_part64);
   hipFree(p->d_part32);
+  tsc::pk_free(p->pk);
   for (int i = 0; i < 3; ++i) {
     if (p->ev[i]) hipEventDestroy(p->ev[i]);
     if (p->gexec[i]) hipGraphExecDestroy(p->gexec[i]);
@@ -402,6 +403,8 @@ int plan_create_impl(tsc_ctx *ctx, const double *ops, const double2 *ops_dev, in
   const tsc_outblock *os[2] = {o1, o2};
   const int32_t ncs[2] = {nc1, nc2}, nos[2] = {no1, no2};
   for (int s = 0; s < 2; ++s) {
+    p->h_c[s].assign(cs[s], cs[s] + ncs[s]);
+    p->h_o[s].assign(os[s], os[s] + nos[s]);
     if ((rc = upload<tsc_contrib>(&p->d_c[s], cs[s], ncs[s], ctx->stream))) return fail(rc);
     if ((rc = upload<tsc_outblock>(&p->d_o[s], os[s], nos[s], ctx->stream))) return fail(rc);
     Units un = make_tiles(cs[s], os[s], nos[s]);
@@ -435,6 +438,8 @@ extern "C" int tsc_plan_update_ops(tsc_plan *p, const double *ops) {
                        p->ops_elems);
     TSC_HIP_CHECK(hipGetLastError());
   }
+  int rc = tsc::pk_repack_ops(p);
+  if (rc) return rc;
   TSC_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
   return TSC_OK;
 }
@@ -450,6 +455,8 @@ extern "C" int tsc_plan_fill_ops_gaussian(tsc_plan *p, int64_t off, int64_t coun
                        p->d_ops32 + off, count);
     TSC_HIP_CHECK(hipGetLastError());
   }
+  rc = tsc::pk_repack_ops(p);
+  if (rc) return rc;
   TSC_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
   return TSC_OK;
 }
@@ -463,6 +470,13 @@ extern "C" int tsc_plan_flops(const tsc_plan *p, double *flops) {
 namespace {
 int ensure_precision(tsc_plan *p, int prec) {
   hipStream_t s = p->ctx->stream;
+  if (prec == TSC_PREC_BF16X3) {
+    if (p->ctx->mfma_rowmode[2] != 0) {
+      set_error("BF16x3 path: this device's 32x32x16 bf16 MFMA accumulator layout is not the one the kernel uses");
+      return TSC_EHIP;
+    }
+    return pk_build(p);
+  }
   const int64_t slots = std::max(p->nslot[0], p->nslot[1]);
   if (slots > 0) {
     void **part = (prec == TSC_PREC_F64) ? &p->d_part64 : &p->d_part32;
@@ -484,6 +498,7 @@ int ensure_precision(tsc_plan *p, int prec) {
 
 int enqueue_hpsi(tsc_plan *p, const double2 *in, double2 *out, int prec, bool events) {
   hipStream_t s = p->ctx->stream;
+  if (prec == TSC_PREC_BF16X3) return pk_enqueue(p, in, out, events);
   Bufs b;
   b.p[TSC_BUF_OPS] = (prec == TSC_PREC_F64) ? (const void *)p->d_ops64 : (const void *)p->d_ops32;
   b.p[TSC_BUF_IN] = in;

@@ -20,8 +20,7 @@
 // accumulators (4 real MFMAs per complex k-group).
 //   FP64  : v_mfma_f64_16x16x4f64, TK = 16
 //   FP32  : v_mfma_f32_16x16x4f32, TK = 16 (complex FP32 products)
-//   BF16x3: v_mfma_f32_16x16x32_bf16 on hi/lo splits, TK = 32
-//           a*b ~ ah*bh + ah*bl + al*bh, FP32 accumulation.
+// The BF16x3 path has kernels of its own (packed operands, tsc_pk.hip).
 #include "tsc_internal.h"
 
 namespace tsc {
@@ -357,137 +356,6 @@ __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outbl
 }
 
 // ---------------------------------------------------------------------
-// BF16x3 kernel: operands are loaded as FP32, split into bf16 hi/lo in LDS.
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
-}
-
-__global__ __launch_bounds__(256) void grouped_kernel_bf16x3(Bufs bufs, const tsc_outblock *__restrict__ blocks,
-                                                             const tsc_contrib *__restrict__ contribs,
-                                                             const Tile *__restrict__ tiles, int rowmode,
-                                                             void *part) {
-  constexpr int TK = 32;
-  constexpr int NA = (TM * TK) / 256, NB = (TN * TK) / 256;
-  // layout [row][k] so that each lane reads 8 consecutive k (16 bytes)
-  constexpr int KP = TK + 8;
-  __shared__ __bf16 Ah_r[TM][KP], Al_r[TM][KP], Ah_i[TM][KP], Al_i[TM][KP];
-  __shared__ __bf16 Bh_r[TN][KP], Bl_r[TN][KP], Bh_i[TN][KP], Bl_i[TN][KP];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = (wave & 1) * 32;
-  const int wn = (wave >> 1) * 32;
-  const Tile t = tiles[blockIdx.x];
-  const tsc_outblock ob = blocks[t.blk];
-  const int i0 = t.ti * TM, j0 = t.tj * TN;
-  const int m = ob.m, n = ob.n;
-
-  f4 accR[2][2], accI[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      accR[a][b] = f4{0, 0, 0, 0};
-      accI[a][b] = f4{0, 0, 0, 0};
-    }
-
-  elementwise_pass<float>(bufs, contribs, t.cbeg, t.cend, i0, j0, m, n, wm, wn, lane, rowmode, accR, accI);
-
-  int ci = next_gemm(contribs, t.cbeg, t.cend);
-  int k0 = 0;
-  Stage<float, NA, NB> st;
-  bool a_tr = false, b_tr = false;
-  if (ci < t.cend) {
-    const tsc_contrib c = contribs[ci];
-    load_stage<float, TK, NA, NB>(bufs, c, 0, i0, j0, m, n, tid, st);
-    a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
-    b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
-  }
-  int Kcur = (ci < t.cend) ? contribs[ci].k : 0;
-  while (ci < t.cend) {
-#pragma unroll
-    for (int r = 0; r < NA; ++r) {
-      int ii, kk;
-      a_coord<TK>(a_tr, tid, r, ii, kk);
-      float pr, pi;
-      cmul<float>(st.cre, st.cim, st.ar[r], st.ai[r], pr, pi);
-      split_bf16(pr, Ah_r[ii][kk], Al_r[ii][kk]);
-      split_bf16(pi, Ah_i[ii][kk], Al_i[ii][kk]);
-    }
-#pragma unroll
-    for (int r = 0; r < NB; ++r) {
-      int jj, kk;
-      b_coord<TK>(b_tr, tid, r, jj, kk);
-      split_bf16(st.br[r], Bh_r[jj][kk], Bl_r[jj][kk]);
-      split_bf16(st.bi[r], Bh_i[jj][kk], Bl_i[jj][kk]);
-    }
-    __syncthreads();
-    int nci = ci, nk0 = k0 + TK;
-    if (nk0 >= Kcur) {
-      nci = next_gemm(contribs, ci + 1, t.cend);
-      nk0 = 0;
-    }
-    if (nci < t.cend) {
-      const tsc_contrib c = contribs[nci];
-      load_stage<float, TK, NA, NB>(bufs, c, nk0, i0, j0, m, n, tid, st);
-      a_tr = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C);
-      b_tr = (c.b_op == TSC_OP_T || c.b_op == TSC_OP_C);
-      if (nci != ci) Kcur = c.k;
-    }
-    // 16x16x32 bf16: lane holds row (lane&15), k = 8*(lane>>4) + 0..7
-    const int kq = 8 * (lane >> 4);
-    bf8 ahr[2], alr[2], ahi[2], ali[2], nahi[2], nali[2], bhr[2], blr[2], bhi[2], bli[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int row = wm + a * 16 + (lane & 15);
-      ahr[a] = *reinterpret_cast<const bf8 *>(&Ah_r[row][kq]);
-      alr[a] = *reinterpret_cast<const bf8 *>(&Al_r[row][kq]);
-      ahi[a] = *reinterpret_cast<const bf8 *>(&Ah_i[row][kq]);
-      ali[a] = *reinterpret_cast<const bf8 *>(&Al_i[row][kq]);
-      nahi[a] = -ahi[a];
-      nali[a] = -ali[a];
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int col = wn + b * 16 + (lane & 15);
-      bhr[b] = *reinterpret_cast<const bf8 *>(&Bh_r[col][kq]);
-      blr[b] = *reinterpret_cast<const bf8 *>(&Bl_r[col][kq]);
-      bhi[b] = *reinterpret_cast<const bf8 *>(&Bh_i[col][kq]);
-      bli[b] = *reinterpret_cast<const bf8 *>(&Bl_i[col][kq]);
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        f4 r = accR[a][b], i = accI[a][b];
-        // Re += ar*br - ai*bi  (3-term split each)
-        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhr[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], blr[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nali[a], bhi[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bli[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhr[b], r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nahi[a], bhi[b], r, 0, 0, 0);
-        // Im += ar*bi + ai*br
-        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alr[a], bhi[b], i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bli[b], i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ali[a], bhr[b], i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], blr[b], i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr[a], bhi[b], i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[a], bhr[b], i, 0, 0, 0);
-        accR[a][b] = r;
-        accI[a][b] = i;
-      }
-    __syncthreads();
-    ci = nci;
-    k0 = nk0;
-  }
-
-  write_unit<float>(bufs, ob, t, part, i0, j0, wm, wn, lane, rowmode, accR, accI);
-}
-
-// ---------------------------------------------------------------------
 // Sum the partial tiles of split units in slot order and write C.
 template <typename CT>
 __global__ __launch_bounds__(256) void reduce_kernel(Bufs bufs, const tsc_outblock *__restrict__ blocks,
@@ -539,7 +407,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(Bufs bufs, const tsc_outblo
 // ---------------------------------------------------------------------
 // Layout probe: C = A B with A[i][k] = i (or 1) and B[k][j] = 1 (or j)
 // for every k; out[kind][pass][lane*4 + r] = C element held in register r,
-// i.e. K*row or K*col of that element. Uses only the row-of-A = lane & 15
+// i.e. 4*row or 4*col of that element. Uses only the row-of-A = lane & 15
 // and col-of-B = lane & 15 conventions of the 16x16 MFMA family.
 __global__ void probe_kernel(float *out) {
   const int lane = threadIdx.x;
@@ -560,40 +428,55 @@ __global__ void probe_kernel(float *out) {
       out[1 * 512 + 256 + lane * 4 + r] = c2[r];
     }
   }
-  {
-    bf8 vi, v1;
-    for (int e = 0; e < 8; ++e) {
-      vi[e] = (__bf16)(float)i;
-      v1[e] = (__bf16)1.0f;
-    }
-    f4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vi, v1, f4{0, 0, 0, 0}, 0, 0, 0);
-    f4 c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v1, vi, f4{0, 0, 0, 0}, 0, 0, 0);
-    for (int r = 0; r < 4; ++r) {
-      out[2 * 512 + lane * 4 + r] = c1[r];
-      out[2 * 512 + 256 + lane * 4 + r] = c2[r];
-    }
+}
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// Row of accumulator register r (0..15) held by `lane` in a 32x32 MFMA
+// output block (the column is lane & 31), as the packed BF16x3 kernels use it.
+__host__ __device__ __forceinline__ int acc_row32(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// 32x32x16 bf16: out[lane*16 + r] = 16*row, out[1024 + lane*16 + r] = 16*col
+__global__ void probe32_kernel(float *out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 31;
+  bf8 vi, v1;
+  for (int e = 0; e < 8; ++e) {
+    vi[e] = (__bf16)(float)i;
+    v1[e] = (__bf16)1.0f;
+  }
+  f16v z;
+  for (int r = 0; r < 16; ++r) z[r] = 0.f;
+  f16v c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vi, v1, z, 0, 0, 0);
+  f16v c2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v1, vi, z, 0, 0, 0);
+  for (int r = 0; r < 16; ++r) {
+    out[lane * 16 + r] = c1[r];
+    out[1024 + lane * 16 + r] = c2[r];
   }
 }
 
+// modes[0], modes[1]: row layout of the f64 / f32 16x16 accumulators;
+// modes[2]: 0 if the 32x32x16 bf16 accumulator layout is the one
+// acc_row32() assumes, -1 otherwise (the BF16x3 path then refuses to run).
 int probe_layouts(hipStream_t stream, int modes[3]) {
   float *d = nullptr;
-  TSC_HIP_CHECK(hipMalloc((void **)&d, sizeof(float) * 3 * 512));
+  TSC_HIP_CHECK(hipMalloc((void **)&d, sizeof(float) * (2 * 512 + 2048)));
   hipLaunchKernelGGL(probe_kernel, dim3(1), dim3(64), 0, stream, d);
+  hipLaunchKernelGGL(probe32_kernel, dim3(1), dim3(64), 0, stream, d + 2 * 512);
   hipError_t le = hipGetLastError();
-  std::vector<float> h(3 * 512);
-  hipError_t ce = hipMemcpyAsync(h.data(), d, sizeof(float) * 3 * 512, hipMemcpyDeviceToHost, stream);
+  std::vector<float> h(2 * 512 + 2048);
+  hipError_t ce = hipMemcpyAsync(h.data(), d, sizeof(float) * h.size(), hipMemcpyDeviceToHost, stream);
   hipError_t se = hipStreamSynchronize(stream);
   hipFree(d);
   TSC_HIP_CHECK(le);
   TSC_HIP_CHECK(ce);
   TSC_HIP_CHECK(se);
-  const float K[3] = {4.f, 4.f, 32.f};
-  for (int kind = 0; kind < 3; ++kind) {
+  for (int kind = 0; kind < 2; ++kind) {
     int ok[2] = {1, 1};
     for (int lane = 0; lane < 64; ++lane)
       for (int r = 0; r < 4; ++r) {
-        const float row = h[kind * 512 + lane * 4 + r] / K[kind];
-        const float col = h[kind * 512 + 256 + lane * 4 + r] / K[kind];
+        const float row = h[kind * 512 + lane * 4 + r] / 4.f;
+        const float col = h[kind * 512 + 256 + lane * 4 + r] / 4.f;
         if (col != (float)(lane & 15)) ok[0] = ok[1] = 0;
         if (row != (float)(4 * (lane >> 4) + r)) ok[0] = 0;
         if (row != (float)((lane >> 4) + 4 * r)) ok[1] = 0;
@@ -607,6 +490,12 @@ int probe_layouts(hipStream_t stream, int modes[3]) {
       return TSC_EHIP;
     }
   }
+  const float *h32 = h.data() + 2 * 512;
+  modes[2] = 0;
+  for (int lane = 0; lane < 64; ++lane)
+    for (int r = 0; r < 16; ++r)
+      if (h32[lane * 16 + r] != 16.f * acc_row32(lane, r) || h32[1024 + lane * 16 + r] != 16.f * (lane & 31))
+        modes[2] = -1;
   return TSC_OK;
 }
 
@@ -622,8 +511,7 @@ hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_bloc
     hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
                        modes[1], part);
   } else {
-    hipLaunchKernelGGL(grouped_kernel_bf16x3, grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles, modes[2],
-                       part);
+    return hipErrorNotSupported;  // BF16x3 runs on the packed-operand kernels
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nreds <= 0) return e;

@@ -57,8 +57,77 @@ hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_bloc
                           const tsc_contrib *d_contribs, const Tile *d_tiles, int ntiles,
                           const Reduce *d_reds, int nreds, void *part, hipStream_t stream,
                           const int *modes);
-// Determine the accumulator row layout of the f64 / f32 / bf16 MFMAs.
+// Determine the accumulator row layout of the f64 / f32 MFMAs and check the
+// 32x32x16 bf16 one.
 int probe_layouts(hipStream_t stream, int modes[3]);
+
+// ---------------------------------------------------------------------------
+// Packed-operand BF16x3 path.
+//
+// Every GEMM operand is read from a "panel": P[row][k] = op(X) restricted to
+// one contribution's window, rows = output rows (A) or output columns (B),
+// stored as bf16 hi/lo splits of the FP32 value. A panel row is a sequence
+// of chunks of 8 k; a chunk holds four 16-byte planes (re hi, im hi, re lo,
+// im lo), so one MFMA fragment (8 consecutive k of one row and plane) is one
+// 16-byte load. k is zero-padded to a multiple of 16 (one MFMA step).
+enum PkBuf { PK_OPS = 0, PK_IN = 1, PK_MID = 2 };
+
+struct PkContrib {
+  int64_t a_off, b_off;    // index (16-byte units) of the first panel row
+  int32_t a_nch, b_nch;    // chunks per panel row (padded k / 8)
+  int32_t a_rmax, b_rmax;  // last panel row; reads are clamped to it
+  int32_t ksteps;          // padded k / 16; 0: no GEMM (elementwise or zero coefficient)
+  int32_t flags;           // bit0 conj A, bit1 conj B, bits 4-5 A PkBuf, bits 6-7 B PkBuf
+  double coef_re, coef_im;
+};
+
+// Packed destinations of one stage-1 (MID) output block: panel as a row
+// operand (r_off, rows = block rows) and as a transposed one (c_off, rows =
+// block columns); -1 when no contribution reads it that way.
+struct PkOut {
+  int64_t r_off, c_off;
+  int32_t r_nch, c_nch;
+};
+
+// One packing job: the chunks [first, first + 256) of one panel.
+struct PkPanel {
+  int64_t src_off;  // element offset of the window in its complex128 buffer
+  int64_t dst;      // index (16-byte units) of the panel
+  int32_t ld;       // leading dimension of the window
+  int32_t rows;     // panel rows
+  int32_t kdim;     // k extent before padding
+  int32_t nch;      // chunks per row
+  int32_t trans;    // 0: P[r][k] = X[r + k*ld]; 1: P[r][k] = X[k + r*ld]
+  int32_t pad;
+};
+struct PkJob {
+  int32_t panel;
+  int32_t pad;
+  int64_t first;
+};
+
+struct PkStage {
+  PkContrib *d_pc = nullptr;   // packed-path contribution records (permuted order)
+  tsc_contrib *d_cp = nullptr; // the same contributions, original records (elementwise terms)
+  PkOut *d_po = nullptr;       // stage-1 blocks: packed destinations
+  Tile *d_t = nullptr;
+  Reduce *d_r = nullptr;
+  int nt = 0, nr = 0;
+  int64_t nslot = 0;
+};
+
+struct PkState {
+  std::string unsupported;  // non-empty: reason the plan cannot run on the packed path
+  void *d_ops = nullptr, *d_in = nullptr, *d_mid = nullptr;
+  int64_t ops_len = 0, in_len = 0, mid_len = 0;  // 16-byte units
+  PkPanel *d_ops_panels = nullptr, *d_in_panels = nullptr;
+  PkJob *d_ops_jobs = nullptr, *d_in_jobs = nullptr;
+  int n_ops_jobs = 0, n_in_jobs = 0;
+  PkStage st[2];
+  void *d_part = nullptr;  // float2 partial tiles
+};
+
+void pk_free(PkState *s);
 
 // Tile shape of the grouped contraction kernels.
 constexpr int kTM = 64;
@@ -118,6 +187,10 @@ struct tsc_plan {
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   const void *g_in[3] = {nullptr, nullptr, nullptr};
   const void *g_out[3] = {nullptr, nullptr, nullptr};
+  // host copies of the task arrays (the packed path is built from them on first use)
+  std::vector<tsc_contrib> h_c[2];
+  std::vector<tsc_outblock> h_o[2];
+  tsc::PkState *pk = nullptr;
 };
 
 namespace tsc {
@@ -126,6 +199,17 @@ int plan_create_impl(tsc_ctx *ctx, const double *ops, const double2 *ops_dev, in
                      int32_t nc2, const tsc_outblock *o2, int32_t no2, int64_t in_elems, int64_t out_elems,
                      tsc_plan **out);
 int hpsi_dev(tsc_plan *p, const double2 *in, double2 *out, int prec);
+// packed BF16x3 path (tsc_pk.hip)
+int pk_build(tsc_plan *p);         // build on first use; packs the operator arena
+int pk_repack_ops(tsc_plan *p);    // after the arena changed
+int pk_enqueue(tsc_plan *p, const double2 *in, double2 *out, bool events);
+// work units of one stage (shared by all paths)
+struct Units {
+  std::vector<Tile> tiles;
+  std::vector<Reduce> reds;
+  int64_t nslot = 0;
+};
+Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no);
 double tridiag_lowest(const double *d, const double *e, int n);
 void tridiag_vector(const double *d, const double *e, int n, double lam, double *z);
 }  // namespace tsc
