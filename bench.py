@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "f32", "f64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the measurement, copy a buffer of this many MiB three times (a kernel of known "
+                         "HBM traffic, to scale hardware byte counters)")
     return ap.parse_args()
 
 
@@ -312,6 +315,15 @@ def main():
     }
     line["config"].update(extra)
     print(json.dumps(line), flush=True)
+    if args.calibrate_copy_mib > 0:
+        n = args.calibrate_copy_mib * (1 << 20) // 16
+        a, b = ctx.vec(n), ctx.vec(n)
+        a.fill_gaussian(1, 2, 1.0)
+        for _ in range(3):
+            b.copy_from(a)
+        ctx.sync()
+        print(f"# copy calibration: {n * 16} bytes read + {n * 16} bytes written per copy", file=sys.stderr,
+              flush=True)
 
 
 if __name__ == "__main__":

@@ -249,8 +249,16 @@ namespace tsc {
 // Work units of one stage: every 64x64 tile of every output block; tiles
 // whose contribution list is long are cut into contiguous chunks of about
 // `target` k-steps that write partial tiles (summed by a Reduce entry).
-// Units are ordered by work, largest first.
-Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no) {
+//
+// Placement: the device hands workgroup b to XCD b % kXcds, and each XCD
+// has its own L2. All units of one output block read the same operand
+// windows, so they form one group that goes to a single XCD, units of the
+// same contribution chunk next to each other; groups are dealt to the XCDs
+// largest first onto the least-loaded one, and the per-XCD lists are
+// interleaved (padded with empty units, blk = -1, that exit at once).
+constexpr int kXcds = 8;  // MI355X: 256 CUs in 8 XCDs
+
+Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm, int tn, double desired) {
   auto ccost = [&](int32_t ci) {
     return (c[ci].a_op == TSC_OP_I || c[ci].b_op == TSC_OP_I) ? 1.0 : (double)std::max(c[ci].k, 1);
   };
@@ -260,51 +268,74 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no) {
     if (o[b].m == 0 || o[b].n == 0) continue;
     double w = 0.0;
     for (int32_t ci = o[b].cbeg; ci < o[b].cend; ++ci) w += ccost(ci);
-    const int64_t nt = (int64_t)((o[b].m + kTM - 1) / kTM) * ((o[b].n + kTN - 1) / kTN);
+    const int64_t nt = (int64_t)((o[b].m + tm - 1) / tm) * ((o[b].n + tn - 1) / tn);
     total += std::max(w, 1.0) * nt;
     ntile += nt;
   }
-  // aim at several units per wavefront slot of the device (256 CUs x 3 WGs)
-  const double desired = 6144.0;
+  // aim at several units per workgroup slot of the device
   const double target = (ntile >= (int64_t)desired) ? 1e300 : std::max(total / desired, 64.0);
   Units u;
-  std::vector<double> cost;
+  struct Group {
+    double cost = 0.0;
+    std::vector<Tile> units;
+  };
+  std::vector<Group> groups;
   for (int32_t b = 0; b < no; ++b) {
     const tsc_outblock &ob = o[b];
     if (ob.m == 0 || ob.n == 0) continue;
     double w = 0.0;
     for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) w += ccost(ci);
-    const int nti = (ob.m + kTM - 1) / kTM, ntj = (ob.n + kTN - 1) / kTN;
-    for (int ti = 0; ti < nti; ++ti)
-      for (int tj = 0; tj < ntj; ++tj) {
-        if (w <= target || ob.cend - ob.cbeg <= 1) {
-          u.tiles.push_back(Tile{b, (int16_t)ti, (int16_t)tj, ob.cbeg, ob.cend, -1, 0});
-          cost.push_back(std::max(w, 1.0));
-          continue;
+    const int nti = (ob.m + tm - 1) / tm, ntj = (ob.n + tn - 1) / tn;
+    std::vector<int32_t> cut{ob.cbeg};
+    std::vector<double> ccut;
+    const bool split = !(w <= target || ob.cend - ob.cbeg <= 1);
+    if (!split) {
+      cut.push_back(ob.cend);
+      ccut.push_back(std::max(w, 1.0));
+    } else {
+      double acc = 0.0;
+      for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) {
+        acc += ccost(ci);
+        if (acc >= target || ci + 1 == ob.cend) {
+          cut.push_back(ci + 1);
+          ccut.push_back(acc);
+          acc = 0.0;
         }
-        Reduce r{b, (int16_t)ti, (int16_t)tj, (int32_t)u.nslot, 0};
-        int32_t start = ob.cbeg;
-        double acc = 0.0;
-        for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) {
-          acc += ccost(ci);
-          if (acc >= target || ci + 1 == ob.cend) {
-            u.tiles.push_back(Tile{b, (int16_t)ti, (int16_t)tj, start, ci + 1, (int32_t)u.nslot, 0});
-            cost.push_back(acc);
-            ++u.nslot;
-            ++r.nslot;
-            start = ci + 1;
-            acc = 0.0;
-          }
-        }
-        u.reds.push_back(r);
       }
+    }
+    const int nch = (int)ccut.size();
+    const int64_t slot_base = u.nslot;
+    if (split) {
+      for (int ti = 0; ti < nti; ++ti)
+        for (int tj = 0; tj < ntj; ++tj)
+          u.reds.push_back(Reduce{b, (int16_t)ti, (int16_t)tj, (int32_t)(slot_base + (int64_t)(ti * ntj + tj) * nch),
+                                  nch});
+      u.nslot += (int64_t)nti * ntj * nch;
+    }
+    Group g;
+    for (int ch = 0; ch < nch; ++ch)
+      for (int ti = 0; ti < nti; ++ti)
+        for (int tj = 0; tj < ntj; ++tj) {
+          const int32_t slot = split ? (int32_t)(slot_base + (int64_t)(ti * ntj + tj) * nch + ch) : -1;
+          g.units.push_back(Tile{b, (int16_t)ti, (int16_t)tj, cut[ch], cut[ch + 1], slot, 0});
+          g.cost += ccut[ch];
+        }
+    groups.push_back(std::move(g));
   }
-  std::vector<size_t> order(u.tiles.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-  std::vector<Tile> sorted(u.tiles.size());
-  for (size_t i = 0; i < order.size(); ++i) sorted[i] = u.tiles[order[i]];
-  u.tiles.swap(sorted);
+  std::stable_sort(groups.begin(), groups.end(), [](const Group &x, const Group &y) { return x.cost > y.cost; });
+  std::vector<std::vector<Tile>> lists(kXcds);
+  std::vector<double> load(kXcds, 0.0);
+  for (const Group &g : groups) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lists[x].insert(lists[x].end(), g.units.begin(), g.units.end());
+    load[x] += g.cost;
+  }
+  size_t len = 0;
+  for (const auto &l : lists) len = std::max(len, l.size());
+  u.tiles.assign(len * kXcds, Tile{-1, 0, 0, 0, 0, -1, 0});
+  for (int x = 0; x < kXcds; ++x)
+    for (size_t i = 0; i < lists[x].size(); ++i) u.tiles[i * kXcds + x] = lists[x][i];
+  while (!u.tiles.empty() && u.tiles.back().blk < 0) u.tiles.pop_back();
   return u;
 }
 }  // namespace tsc

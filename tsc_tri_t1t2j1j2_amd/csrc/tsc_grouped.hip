@@ -259,6 +259,7 @@ __global__ __launch_bounds__(256) void grouped_kernel(Bufs bufs, const tsc_outbl
   const int wm = (wave & 1) * 32;
   const int wn = (wave >> 1) * 32;
   const Tile t = tiles[blockIdx.x];
+  if (t.blk < 0) return;  // padding unit
   const tsc_outblock ob = blocks[t.blk];
   const int i0 = t.ti * TM, j0 = t.tj * TN;
   const int m = ob.m, n = ob.n;

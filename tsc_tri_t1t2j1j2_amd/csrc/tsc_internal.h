@@ -66,19 +66,26 @@ int probe_layouts(hipStream_t stream, int modes[3]);
 //
 // Every GEMM operand is read from a "panel": P[row][k] = op(X) restricted to
 // one contribution's window, rows = output rows (A) or output columns (B),
-// stored as bf16 hi/lo splits of the FP32 value. A panel row is a sequence
-// of chunks of 8 k; a chunk holds four 16-byte planes (re hi, im hi, re lo,
-// im lo), so one MFMA fragment (8 consecutive k of one row and plane) is one
-// 16-byte load. k is zero-padded to a multiple of 16 (one MFMA step).
+// stored as bf16 hi/lo splits of the FP32 value. k is zero-padded to a
+// multiple of 16 (one MFMA step) and cut into chunks of 8; rows are padded
+// to a multiple of 64. Storage unit: an "entry" of 16 bytes = 8 bf16 = one
+// MFMA fragment (8 consecutive k of one row and plane). A panel is a
+// sequence of 1 KB pieces, piece (rb, c, p) = the entries of rows
+// 64*rb .. 64*rb+63 for chunk c and plane p (0 re hi, 1 im hi, 2 re lo,
+// 3 im lo):
+//     entry(r, c, p) = ((r / 64 * nch + c) * 4 + p) * 64 + r % 64
+// so a workgroup moves one step's operands with contiguous 1 KB loads and
+// a half-wave reads one fragment set from 512 contiguous LDS bytes.
 enum PkBuf { PK_OPS = 0, PK_IN = 1, PK_MID = 2 };
+constexpr int kPkTile = 128;  // output tile of one packed-path workgroup (4 waves of 64x64)
 
 struct PkContrib {
-  int64_t a_off, b_off;    // index (16-byte units) of the first panel row
-  int32_t a_nch, b_nch;    // chunks per panel row (padded k / 8)
-  int32_t a_rmax, b_rmax;  // last panel row; reads are clamped to it
-  int32_t ksteps;          // padded k / 16; 0: no GEMM (elementwise or zero coefficient)
-  int32_t flags;           // bit0 conj A, bit1 conj B, bits 4-5 A PkBuf, bits 6-7 B PkBuf
-  double coef_re, coef_im;
+  int64_t a_off, b_off;  // entry index of the operand panels
+  int32_t a_nch, b_nch;  // chunks per panel row (padded k / 8)
+  int32_t ksteps;        // padded k / 16; 0: no GEMM (elementwise or zero coefficient)
+  int32_t flags;         // bit0 conj A, bit1 conj B, bits 4-5 A PkBuf, bits 6-7 B PkBuf
+  float rs_re, rs_im;    // coefficient of the previous GEMM contribution / this one
+  float g_re, g_im;      // this contribution's coefficient
 };
 
 // Packed destinations of one stage-1 (MID) output block: panel as a row
@@ -89,12 +96,13 @@ struct PkOut {
   int32_t r_nch, c_nch;
 };
 
-// One packing job: the chunks [first, first + 256) of one panel.
+// One packing job: the (row, chunk) items [first, first + 256) of one panel,
+// item = chunk * padded_rows + row.
 struct PkPanel {
   int64_t src_off;  // element offset of the window in its complex128 buffer
-  int64_t dst;      // index (16-byte units) of the panel
+  int64_t dst;      // entry index of the panel
   int32_t ld;       // leading dimension of the window
-  int32_t rows;     // panel rows
+  int32_t rows;     // panel rows (before padding to 64)
   int32_t kdim;     // k extent before padding
   int32_t nch;      // chunks per row
   int32_t trans;    // 0: P[r][k] = X[r + k*ld]; 1: P[r][k] = X[k + r*ld]
@@ -119,7 +127,7 @@ struct PkStage {
 struct PkState {
   std::string unsupported;  // non-empty: reason the plan cannot run on the packed path
   void *d_ops = nullptr, *d_in = nullptr, *d_mid = nullptr;
-  int64_t ops_len = 0, in_len = 0, mid_len = 0;  // 16-byte units
+  int64_t ops_len = 0, in_len = 0, mid_len = 0;  // entries
   PkPanel *d_ops_panels = nullptr, *d_in_panels = nullptr;
   PkJob *d_ops_jobs = nullptr, *d_in_jobs = nullptr;
   int n_ops_jobs = 0, n_in_jobs = 0;
@@ -209,7 +217,8 @@ struct Units {
   std::vector<Reduce> reds;
   int64_t nslot = 0;
 };
-Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no);
+Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm = kTM, int tn = kTN,
+                 double desired_units = 6144.0);
 double tridiag_lowest(const double *d, const double *e, int n);
 void tridiag_vector(const double *d, const double *e, int n, double lam, double *z);
 }  // namespace tsc

@@ -9,12 +9,15 @@
 //  * every operand is read from a pre-split panel (layout in tsc_internal.h):
 //    the operator arena is packed once per plan, the psi windows once per
 //    H.psi call, and stage 1 writes its intermediates directly as panels;
-//    the contraction kernel therefore does no conversion work and loads each
-//    MFMA fragment with one 16-byte load straight into registers;
-//  * one wave owns one 64x64 output unit as 2x2 blocks of 32x32 MFMAs
-//    (no LDS, no barriers); the MFMA is issued with the B panel as its first
-//    operand, so a lane's accumulators run along output columns and stores
-//    of psi / panels are contiguous across lanes;
+//    the contraction kernel does no conversion work;
+//  * a workgroup of 4 waves owns one 128x128 output unit, one 64x64 quarter
+//    per wave as 2x2 blocks of 32x32 MFMAs; each 16-k step of A (128 rows)
+//    and B (128 rows) is moved once into LDS with contiguous 1 KB loads
+//    (double-buffered: the loads of step s+1 are in flight while step s is
+//    computed) and shared by the two waves that need each half;
+//  * the MFMA is issued with the B fragment as its first operand, so a
+//    lane's accumulators run along output columns and stores of psi /
+//    panels are contiguous across lanes;
 //  * the contributions of a unit are ordered by coefficient; the
 //    accumulators hold (partial sum) / (current coefficient) and are rescaled
 //    when the coefficient changes, so coefficients never touch the operands.
@@ -34,7 +37,11 @@ typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int TM = kTM, TN = kTN;
+constexpr int WT = kPkTile;  // 128
+
+__host__ __device__ __forceinline__ int64_t entry(int r, int c, int p, int nch) {
+  return (((int64_t)(r >> 6) * nch + c) * 4 + p) * 64 + (r & 63);
+}
 
 __device__ __forceinline__ void split2(float x, __bf16 &hi, __bf16 &lo) {
   hi = (__bf16)x;
@@ -48,31 +55,25 @@ __device__ __forceinline__ bf8 neg8(bf8 x) {
 }
 
 // ---------------------------------------------------------------------------
-// Packing: one thread per (row, chunk) of a panel. Row-operand panels
-// (trans 0) walk rows fastest so the strided reads of a column-major window
-// coalesce; transposed panels read 8 consecutive elements per thread.
+// Packing: one thread per (row, chunk) of a panel, rows fastest (padding rows
+// up to the next multiple of 64 are written as zeros), so the 16-byte entry
+// stores of consecutive threads are contiguous.
 __global__ __launch_bounds__(256) void pack_kernel(const double2 *__restrict__ src, bf8 *__restrict__ dst,
                                                    const PkPanel *__restrict__ panels,
                                                    const PkJob *__restrict__ jobs) {
   const PkJob jb = jobs[blockIdx.x];
   const PkPanel P = panels[jb.panel];
+  const int rows_p = (P.rows + 63) & ~63;
   const int64_t t = jb.first + threadIdx.x;
-  if (t >= (int64_t)P.rows * P.nch) return;
-  int r, c;
-  if (P.trans == 0) {
-    c = (int)(t / P.rows);
-    r = (int)(t % P.rows);
-  } else {
-    r = (int)(t / P.nch);
-    c = (int)(t % P.nch);
-  }
+  if (t >= (int64_t)rows_p * P.nch) return;
+  const int c = (int)(t / rows_p), r = (int)(t % rows_p);
   const double2 *X = src + P.src_off;
   bf8 v0, v1, v2, v3;
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int k = 8 * c + e;
     double2 x = make_double2(0.0, 0.0);
-    if (k < P.kdim) x = P.trans ? X[(int64_t)k + (int64_t)r * P.ld] : X[(int64_t)r + (int64_t)k * P.ld];
+    if (r < P.rows && k < P.kdim) x = P.trans ? X[(int64_t)k + (int64_t)r * P.ld] : X[(int64_t)r + (int64_t)k * P.ld];
     __bf16 a, b, cc, d;
     split2((float)x.x, a, b);
     split2((float)x.y, cc, d);
@@ -81,31 +82,32 @@ __global__ __launch_bounds__(256) void pack_kernel(const double2 *__restrict__ s
     v1[e] = cc;
     v3[e] = d;
   }
-  bf8 *q = dst + P.dst + ((int64_t)r * P.nch + c) * 4;
-  q[0] = v0;
-  q[1] = v1;
-  q[2] = v2;
-  q[3] = v3;
+  bf8 *q = dst + P.dst;
+  q[entry(r, c, 0, P.nch)] = v0;
+  q[entry(r, c, 1, P.nch)] = v1;
+  q[entry(r, c, 2, P.nch)] = v2;
+  q[entry(r, c, 3, P.nch)] = v3;
 }
 
 // ---------------------------------------------------------------------------
 // element helpers for packed intermediates
 __device__ __forceinline__ float unsplit(__bf16 hi, __bf16 lo) { return (float)hi + (float)lo; }
 
-// one element (plane position `pos` of the chunk at 16-byte index idx)
-__device__ __forceinline__ void store_pk_elem(bf8 *mid, int64_t idx, int pos, float re, float im, bool acc) {
-  __bf16 *q = reinterpret_cast<__bf16 *>(mid + idx);
+// one element of a panel: row r, k, at panel entry base `pan`
+__device__ __forceinline__ void store_pk_elem(bf8 *pan, int nch, int r, int k, float re, float im, bool acc) {
+  __bf16 *q0 = reinterpret_cast<__bf16 *>(pan + entry(r, k >> 3, 0, nch)) + (k & 7);
+  // planes are 64 entries (512 bf16) apart
   if (acc) {
-    re += unsplit(q[pos], q[16 + pos]);
-    im += unsplit(q[8 + pos], q[24 + pos]);
+    re += unsplit(q0[0], q0[1024]);
+    im += unsplit(q0[512], q0[1536]);
   }
   __bf16 a, b, c, d;
   split2(re, a, b);
   split2(im, c, d);
-  q[pos] = a;
-  q[8 + pos] = c;
-  q[16 + pos] = b;
-  q[24 + pos] = d;
+  q0[0] = a;
+  q0[512] = c;
+  q0[1024] = b;
+  q0[1536] = d;
 }
 
 struct PkPtrs {
@@ -141,24 +143,49 @@ __device__ __forceinline__ void scale_acc(f16v (&R)[2][2], f16v (&I)[2][2], floa
   }
 }
 
+__device__ __forceinline__ int next_gemm(const PkContrib *__restrict__ pcs, int ci, int cend) {
+  while (ci < cend && pcs[ci].ksteps == 0) ++ci;
+  return ci;
+}
+
 #define MFMA32(x, y, acc) __builtin_amdgcn_mfma_f32_32x32x16_bf16((x), (y), (acc), 0, 0, 0)
 
 // ---------------------------------------------------------------------------
-// Contraction kernel: one wave, one 64x64 unit. Accumulator element r of
-// block (a, b) held by `lane` is output (i0 + 32a + lane%32,
-// j0 + 32b + 8(r/4) + 4(lane/32) + r%4).
-__global__ __launch_bounds__(64, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restrict__ mid, const double2 *__restrict__ in64,
-                                                     const double2 *__restrict__ ops64, double2 *__restrict__ out64,
-                                                     const tsc_outblock *__restrict__ blocks,
-                                                     const PkOut *__restrict__ pouts,
-                                                     const tsc_contrib *__restrict__ cps,
-                                                     const PkContrib *__restrict__ pcs,
-                                                     const Tile *__restrict__ tiles, float2 *__restrict__ part) {
-  const int lane = threadIdx.x, l32 = lane & 31, h = lane >> 5;
+// Contraction kernel: one 128x128 unit per workgroup of 4 waves. Wave w owns
+// the quarter (wa, wb) = (w & 1, w >> 1); accumulator element r of block
+// (a, b) held by `lane` is output (i0 + 64wa + 32a + lane%32,
+// j0 + 64wb + 32b + 8(r/4) + 4(lane/32) + r%4).
+// LDS holds two step buffers of 32 pieces (1 KB each): pieces 0-15 are A
+// (row half rb, chunk h, plane p at 8rb + 4h + p), 16-31 the same for B.
+// Waves 0/1 load the two A halves of a step, waves 2/3 the two B halves.
+// Per step: read this step's fragments from LDS, start the direct
+// global->LDS loads of the next step into the other buffer, run the 48
+// MFMAs, wait for the own loads, workgroup barrier.
+__global__ __launch_bounds__(256, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restrict__ mid,
+                                                         const double2 *__restrict__ in64,
+                                                         const double2 *__restrict__ ops64,
+                                                         double2 *__restrict__ out64,
+                                                         const tsc_outblock *__restrict__ blocks,
+                                                         const PkOut *__restrict__ pouts,
+                                                         const tsc_contrib *__restrict__ cps,
+                                                         const PkContrib *__restrict__ pcs,
+                                                         const Tile *__restrict__ tiles, float2 *__restrict__ part) {
+  __shared__ bf8 lds0[32][64], lds1[32][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, l32 = lane & 31, h = lane >> 5;
+  const int wa = wave & 1, wb = wave >> 1;
   const Tile t = tiles[blockIdx.x];
+  if (t.blk < 0) return;  // padding unit
   const tsc_outblock ob = blocks[t.blk];
-  const int i0 = t.ti * TM, j0 = t.tj * TN;
   const int m = ob.m, n = ob.n;
+  const int i0 = t.ti * WT, j0 = t.tj * WT;
+  const int iw = i0 + 64 * wa, jw = j0 + 64 * wb;
+  const bool active = iw < m && jw < n;
+  // loader role of this wave
+  const bool ldA = wave < 2;
+  const int lrb = wave & 1;
+  const int grb = (ldA ? 2 * t.ti : 2 * t.tj) + lrb;  // 64-row block of the panel
+  const bool ld_on = 64 * grb < (ldA ? m : n);
+  const int q0 = (ldA ? 0 : 16) + 8 * lrb;
 
   f16v accR[2][2], accI[2][2];
 #pragma unroll
@@ -171,77 +198,124 @@ __global__ __launch_bounds__(64, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restric
         accI[a][b][r] = 0.f;
       }
 
-  double g_re = 1.0, g_im = 0.0;  // accumulators hold sum / g
-  for (int ci = t.cbeg; ci < t.cend; ++ci) {
-    const PkContrib c = pcs[ci];
-    if (c.ksteps == 0) continue;
-    if (c.coef_re != g_re || c.coef_im != g_im) {
-      const double den = c.coef_re * c.coef_re + c.coef_im * c.coef_im;
-      const double rr = (g_re * c.coef_re + g_im * c.coef_im) / den;
-      const double ri = (g_im * c.coef_re - g_re * c.coef_im) / den;
-      scale_acc(accR, accI, (float)rr, (float)ri);
-      g_re = c.coef_re;
-      g_im = c.coef_im;
-    }
-    const bf8 *pa = P.p[(c.flags >> 4) & 3] + c.a_off;
-    const bf8 *pb = P.p[(c.flags >> 6) & 3] + c.b_off;
-    const bf8 *ra[2], *rb[2];
+  // one step's operands go straight from the panels into LDS (no registers):
+  // lane l of the wave writes LDS bytes [16 l, 16 l + 16) of each 1 KB piece
+  auto issue_step = [&](const PkContrib &c, int s, bf8 (*L)[64]) {
+    if (!ld_on) return;
+    const int nch = ldA ? c.a_nch : c.b_nch;
+    const bf8 *base = P.p[ldA ? ((c.flags >> 4) & 3) : ((c.flags >> 6) & 3)] + (ldA ? c.a_off : c.b_off) +
+                      ((int64_t)grb * nch + 2 * s) * 256 + lane;
 #pragma unroll
-    for (int a = 0; a < 2; ++a) ra[a] = pa + ((int64_t)min(i0 + 32 * a + l32, c.a_rmax) * c.a_nch + h) * 4;
-#pragma unroll
-    for (int b = 0; b < 2; ++b) rb[b] = pb + ((int64_t)min(j0 + 32 * b + l32, c.b_rmax) * c.b_nch + h) * 4;
-    const bool cja = (c.flags & 1) != 0, cjb = (c.flags & 2) != 0;
-    for (int s = 0; s < c.ksteps; ++s) {
-      bf8 A[2][4], B[2][4];
+    for (int q = 0; q < 8; ++q)  // chunk 2s + q/4, plane q%4
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(base + q * 64),
+                                       (__attribute__((address_space(3))) void *)&L[q0 + q][0], 16, 0, 0);
+  };
+  // all of this wave's loads have landed in LDS; then every wave's have
+  auto step_sync = [&]() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
+  };
+  auto compute = [&](const PkContrib &c, bf8 (*L)[64], bf8 (*Lnext)[64], bool more, const PkContrib &nc, int ns) {
+    bf8 A[2][4], B[2][4];
+    if (active) {
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) A[a][q] = ra[a][8 * s + q];
+        for (int q = 0; q < 4; ++q) A[a][q] = L[8 * wa + 4 * h + q][32 * a + l32];
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) B[b][q] = rb[b][8 * s + q];
-      if (cja) {
+        for (int q = 0; q < 4; ++q) B[b][q] = L[16 + 8 * wb + 4 * h + q][32 * b + l32];
+    }
+    // fragments are in registers: the next step may now be loaded into the other buffer
+    if (more) issue_step(nc, ns, Lnext);
+    if (!active) return;
+    if (c.flags & 1) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          A[a][1] = neg8(A[a][1]);
-          A[a][3] = neg8(A[a][3]);
-        }
+      for (int a = 0; a < 2; ++a) {
+        A[a][1] = neg8(A[a][1]);
+        A[a][3] = neg8(A[a][3]);
       }
-      if (cjb) {
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          B[b][1] = neg8(B[b][1]);
-          B[b][3] = neg8(B[b][3]);
-        }
-      }
-      bf8 NB[2][2];
+    }
+    if (c.flags & 2) {
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        NB[b][0] = neg8(B[b][1]);
-        NB[b][1] = neg8(B[b][3]);
+        B[b][1] = neg8(B[b][1]);
+        B[b][3] = neg8(B[b][3]);
       }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const bf8 nbh = neg8(B[b][1]), nbl = neg8(B[b][3]);
       // planes: 0 re hi, 1 im hi, 2 re lo, 3 im lo
       // Re += br*ar - bi*ai ; Im += bi*ar + br*ai (each product in three terms)
-#define BLOCKS(stmt)                    \
-  _Pragma("unroll") for (int a = 0; a < 2; ++a) \
-      _Pragma("unroll") for (int b = 0; b < 2; ++b) stmt
-      BLOCKS(accR[a][b] = MFMA32(B[b][0], A[a][0], accR[a][b]));
-      BLOCKS(accI[a][b] = MFMA32(B[b][1], A[a][0], accI[a][b]));
-      BLOCKS(accR[a][b] = MFMA32(B[b][2], A[a][0], accR[a][b]));
-      BLOCKS(accI[a][b] = MFMA32(B[b][3], A[a][0], accI[a][b]));
-      BLOCKS(accR[a][b] = MFMA32(B[b][0], A[a][2], accR[a][b]));
-      BLOCKS(accI[a][b] = MFMA32(B[b][1], A[a][2], accI[a][b]));
-      BLOCKS(accR[a][b] = MFMA32(NB[b][0], A[a][1], accR[a][b]));
-      BLOCKS(accI[a][b] = MFMA32(B[b][0], A[a][1], accI[a][b]));
-      BLOCKS(accR[a][b] = MFMA32(NB[b][1], A[a][1], accR[a][b]));
-      BLOCKS(accI[a][b] = MFMA32(B[b][2], A[a][1], accI[a][b]));
-      BLOCKS(accR[a][b] = MFMA32(NB[b][0], A[a][3], accR[a][b]));
-      BLOCKS(accI[a][b] = MFMA32(B[b][0], A[a][3], accI[a][b]));
-#undef BLOCKS
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        accR[a][b] = MFMA32(B[b][0], A[a][0], accR[a][b]);
+        accI[a][b] = MFMA32(B[b][1], A[a][0], accI[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        accR[a][b] = MFMA32(B[b][2], A[a][0], accR[a][b]);
+        accI[a][b] = MFMA32(B[b][3], A[a][0], accI[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        accR[a][b] = MFMA32(B[b][0], A[a][2], accR[a][b]);
+        accI[a][b] = MFMA32(B[b][1], A[a][2], accI[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        accR[a][b] = MFMA32(nbh, A[a][1], accR[a][b]);
+        accI[a][b] = MFMA32(B[b][0], A[a][1], accI[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        accR[a][b] = MFMA32(nbl, A[a][1], accR[a][b]);
+        accI[a][b] = MFMA32(B[b][2], A[a][1], accI[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        accR[a][b] = MFMA32(nbh, A[a][3], accR[a][b]);
+        accI[a][b] = MFMA32(B[b][0], A[a][3], accI[a][b]);
+      }
     }
+  };
+
+  int cc = next_gemm(pcs, t.cbeg, t.cend);
+  PkContrib cur{};
+  if (cc < t.cend) {
+    cur = pcs[cc];
+    issue_step(cur, 0, lds0);
   }
-  if (g_re != 1.0 || g_im != 0.0) scale_acc(accR, accI, (float)g_re, (float)g_im);
+  step_sync();
+  int cs = 0, st = 0;
+  bool first = true;
+  float gr = 1.f, gi = 0.f;
+  while (cc < t.cend) {
+    int nc = cc, ns = cs + 1;
+    PkContrib nxt = cur;
+    if (ns >= cur.ksteps) {
+      nc = next_gemm(pcs, cc + 1, t.cend);
+      ns = 0;
+      if (nc < t.cend) nxt = pcs[nc];
+    }
+    const bool more = nc < t.cend;
+    if (cs == 0) {
+      if (!first && (cur.rs_re != 1.f || cur.rs_im != 0.f)) scale_acc(accR, accI, cur.rs_re, cur.rs_im);
+      first = false;
+      gr = cur.g_re;
+      gi = cur.g_im;
+    }
+    compute(cur, st ? lds1 : lds0, st ? lds0 : lds1, more, nxt, ns);
+    step_sync();
+    cc = nc;
+    cs = ns;
+    cur = nxt;
+    st ^= 1;
+  }
+  if (!active) return;
+  if (gr != 1.f || gi != 0.f) scale_acc(accR, accI, gr, gi);
 
   // elementwise contributions (one operand the identity), FP32 accumulation
   for (int ci = t.cbeg; ci < t.cend; ++ci) {
@@ -261,8 +335,8 @@ __global__ __launch_bounds__(64, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restric
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int i = i0 + 32 * a + l32;
-          const int j = j0 + 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
+          const int i = iw + 32 * a + l32;
+          const int j = jw + 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
           if (i < m && j < n) {
             const double2 x = ld_op64(X, off, ldx, op, i, j);
             const float xr = (float)x.x, xi = (float)x.y;
@@ -272,17 +346,17 @@ __global__ __launch_bounds__(64, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restric
         }
   }
 
-  // ---- write the unit
+  // ---- write this wave's quarter
   if (t.slot >= 0) {
-    float2 *Pp = part + (int64_t)t.slot * (TM * TN);
+    float2 *Pp = part + (int64_t)t.slot * (WT * WT);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int il = 32 * a + l32, jl = 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
-          if (i0 + il < m && j0 + jl < n) Pp[jl * TM + il] = make_float2(accR[a][b][r], accI[a][b][r]);
+          const int il = 64 * wa + 32 * a + l32, jl = 64 * wb + 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
+          if (i0 + il < m && j0 + jl < n) Pp[jl * WT + il] = make_float2(accR[a][b][r], accI[a][b][r]);
         }
     return;
   }
@@ -293,7 +367,7 @@ __global__ __launch_bounds__(64, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restric
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int i = i0 + 32 * a + l32, j = j0 + 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
+          const int i = iw + 32 * a + l32, j = jw + 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
           if (i >= m || j >= n) continue;
           const int64_t idx = ob.c_off + (int64_t)i + (int64_t)j * ob.ldc;
           double2 v = make_double2((double)accR[a][b][r], (double)accI[a][b][r]);
@@ -308,27 +382,28 @@ __global__ __launch_bounds__(64, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restric
   }
   const PkOut po = pouts[t.blk];
   if (po.r_off >= 0) {
-    // row panel: P[i][j]; the 4 consecutive j of (a, b, q) are half a chunk
+    // row panel P[i][j]: the 4 consecutive j of (a, b, q) are half of one entry
     const int kp = 8 * po.r_nch;
+    bf8 *pan = mid + po.r_off;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int i = i0 + 32 * a + l32;
-          const int jb = j0 + 32 * b + 8 * q + 4 * h;
+          const int i = iw + 32 * a + l32;
+          const int jb = jw + 32 * b + 8 * q + 4 * h;
           if (i >= m || jb >= kp) continue;
+          bf4v *dst = reinterpret_cast<bf4v *>(pan + entry(i, jb >> 3, 0, po.r_nch)) + h;
+          // planes are 64 entries = 128 bf4v apart
           bf4v p0, p1, p2, p3;
-          bf4v *dst = reinterpret_cast<bf4v *>(mid + po.r_off + ((int64_t)i * po.r_nch + (jb >> 3)) * 4) + h;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             float vr = (jb + e < n) ? accR[a][b][4 * q + e] : 0.f;
             float vi = (jb + e < n) ? accI[a][b][4 * q + e] : 0.f;
             if (ob.mode) {
-              const bf4v o0 = dst[0], o1 = dst[2], o2 = dst[4], o3 = dst[6];
-              vr += unsplit(o0[e], o2[e]);
-              vi += unsplit(o1[e], o3[e]);
+              vr += unsplit(dst[0][e], dst[256][e]);
+              vi += unsplit(dst[128][e], dst[384][e]);
             }
             __bf16 x0, x1, x2, x3;
             split2(vr, x0, x2);
@@ -338,27 +413,27 @@ __global__ __launch_bounds__(64, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restric
             p2[e] = x2;
             p3[e] = x3;
           }
-          // plane stride is 16 bytes = 2 bf4v
           dst[0] = p0;
-          dst[2] = p1;
-          dst[4] = p2;
-          dst[6] = p3;
+          dst[128] = p1;
+          dst[256] = p2;
+          dst[384] = p3;
         }
   }
   if (po.c_off >= 0) {
-    // transposed panel: P[j][i]
+    // transposed panel P[j][i]
     const int kp = 8 * po.c_nch;
+    bf8 *pan = mid + po.c_off;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int i = i0 + 32 * a + l32, j = j0 + 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
+          const int i = iw + 32 * a + l32, j = jw + 32 * b + 8 * (r >> 2) + 4 * h + (r & 3);
           if (j >= n || i >= kp) continue;
           const bool in_blk = i < m;
-          store_pk_elem(mid, po.c_off + ((int64_t)j * po.c_nch + (i >> 3)) * 4, i & 7,
-                        in_blk ? accR[a][b][r] : 0.f, in_blk ? accI[a][b][r] : 0.f, ob.mode != 0);
+          store_pk_elem(pan, po.c_nch, j, i, in_blk ? accR[a][b][r] : 0.f, in_blk ? accI[a][b][r] : 0.f,
+                        ob.mode != 0);
         }
   }
 }
@@ -372,18 +447,18 @@ __global__ __launch_bounds__(256) void pk_reduce_kernel(bf8 *__restrict__ mid, d
                                                         const float2 *__restrict__ part) {
   const Reduce rd = reds[blockIdx.x];
   const tsc_outblock ob = blocks[rd.blk];
-  const int i0 = rd.ti * TM, j0 = rd.tj * TN;
+  const int i0 = rd.ti * WT, j0 = rd.tj * WT;
   const bool packed = (ob.c_buf == TSC_BUF_MID);
   PkOut po{-1, -1, 0, 0};
   if (packed) po = pouts[rd.blk];
-  for (int e = threadIdx.x; e < TM * TN; e += 256) {
-    const int il = e % TM, jl = e / TM;
+  for (int e = threadIdx.x; e < WT * WT; e += 256) {
+    const int il = e % WT, jl = e / WT;
     const int i = i0 + il, j = j0 + jl;
     const bool inside = (i < ob.m && j < ob.n);
     double sr = 0.0, si = 0.0;
     if (inside) {
       for (int s = 0; s < rd.nslot; ++s) {
-        const float2 v = part[(int64_t)(rd.slot0 + s) * (TM * TN) + e];
+        const float2 v = part[(int64_t)(rd.slot0 + s) * (WT * WT) + e];
         sr += (double)v.x;
         si += (double)v.y;
       }
@@ -401,11 +476,11 @@ __global__ __launch_bounds__(256) void pk_reduce_kernel(bf8 *__restrict__ mid, d
       continue;
     }
     if (po.r_off >= 0 && i < ob.m && j < 8 * po.r_nch)
-      store_pk_elem(mid, po.r_off + ((int64_t)i * po.r_nch + (j >> 3)) * 4, j & 7, inside ? (float)sr : 0.f,
-                    inside ? (float)si : 0.f, ob.mode != 0);
+      store_pk_elem(mid + po.r_off, po.r_nch, i, j, inside ? (float)sr : 0.f, inside ? (float)si : 0.f,
+                    ob.mode != 0);
     if (po.c_off >= 0 && j < ob.n && i < 8 * po.c_nch)
-      store_pk_elem(mid, po.c_off + ((int64_t)j * po.c_nch + (i >> 3)) * 4, i & 7, inside ? (float)sr : 0.f,
-                    inside ? (float)si : 0.f, ob.mode != 0);
+      store_pk_elem(mid + po.c_off, po.c_nch, j, i, inside ? (float)sr : 0.f, inside ? (float)si : 0.f,
+                    ob.mode != 0);
   }
 }
 
@@ -432,10 +507,12 @@ int alloc_bytes(void **p, int64_t bytes, const char *what) {
   return TSC_OK;
 }
 
+int64_t rows64(int rows) { return (rows + 63) & ~63; }
+
 std::vector<PkJob> make_jobs(const std::vector<PkPanel> &pan) {
   std::vector<PkJob> jobs;
   for (size_t i = 0; i < pan.size(); ++i) {
-    const int64_t total = (int64_t)pan[i].rows * pan[i].nch;
+    const int64_t total = rows64(pan[i].rows) * pan[i].nch;
     for (int64_t f = 0; f < total; f += 256) jobs.push_back(PkJob{(int32_t)i, 0, f});
   }
   return jobs;
@@ -502,7 +579,7 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
     P.kdim = kdim;
     P.nch = chunks_for(kdim);
     P.trans = trans;
-    plen[which] += (int64_t)rows * P.nch * 4;
+    plen[which] += rows64(rows) * P.nch * 4;
     pan[which].push_back(P);
     win[which][key] = P.dst;
     return P.dst;
@@ -545,13 +622,16 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
         if (C[x].coef_re != C[y].coef_re) return C[x].coef_re < C[y].coef_re;
         return C[x].coef_im < C[y].coef_im;
       });
+      double prev_re = 0.0, prev_im = 0.0;
       for (size_t q = 0; q < order.size(); ++q) {
         const int64_t pos = ob.cbeg + (int64_t)q;
         const tsc_contrib &c = C[order[q]];
         cps[s][pos] = c;
         PkContrib &pc = pcs[s][pos];
-        pc.coef_re = c.coef_re;
-        pc.coef_im = c.coef_im;
+        pc.g_re = (float)c.coef_re;
+        pc.g_im = (float)c.coef_im;
+        pc.rs_re = 1.f;
+        pc.rs_im = 0.f;
         const int k = c.k;
         if (c.a_op == TSC_OP_I || c.b_op == TSC_OP_I) {
           const int buf = (c.b_op == TSC_OP_I) ? c.a_buf : c.b_buf;
@@ -562,6 +642,14 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
         }
         if (cls(order[q]) == 2 || k <= 0 || ob.m <= 0 || ob.n <= 0) continue;
         pc.ksteps = (k + 15) / 16;
+        // rescale from the previous GEMM contribution of this block (in this order)
+        if (prev_re != 0.0 || prev_im != 0.0) {
+          const double den = c.coef_re * c.coef_re + c.coef_im * c.coef_im;
+          pc.rs_re = (float)((prev_re * c.coef_re + prev_im * c.coef_im) / den);
+          pc.rs_im = (float)((prev_im * c.coef_re - prev_re * c.coef_im) / den);
+        }
+        prev_re = c.coef_re;
+        prev_im = c.coef_im;
         // A panel: rows = output rows; B panel: rows = output columns
         const int ta = (c.a_op == TSC_OP_T || c.a_op == TSC_OP_C) ? 1 : 0;
         const int tb = (c.b_op == TSC_OP_N || c.b_op == TSC_OP_CONJ) ? 1 : 0;
@@ -597,11 +685,9 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
           if (w == 0) {
             pc.a_off = dst;
             pc.a_nch = chunks_for(k);
-            pc.a_rmax = rows[w] - 1;
           } else {
             pc.b_off = dst;
             pc.b_nch = chunks_for(k);
-            pc.b_rmax = rows[w] - 1;
           }
           kinds[w] = kind;
         }
@@ -616,12 +702,12 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
     if (need_r[b]) {
       pouts[b].r_nch = chunks_for(ob.n);
       pouts[b].r_off = S->mid_len;
-      S->mid_len += (int64_t)ob.m * pouts[b].r_nch * 4;
+      S->mid_len += rows64(ob.m) * pouts[b].r_nch * 4;
     }
     if (need_c[b]) {
       pouts[b].c_nch = chunks_for(ob.m);
       pouts[b].c_off = S->mid_len;
-      S->mid_len += (int64_t)ob.n * pouts[b].c_nch * 4;
+      S->mid_len += rows64(ob.n) * pouts[b].c_nch * 4;
     }
   }
   for (const MidRef &r : midrefs) {
@@ -641,6 +727,8 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
   if ((rc = alloc_bytes(&S->d_ops, S->ops_len * 16, "packed arena"))) return rc;
   if ((rc = alloc_bytes(&S->d_in, S->in_len * 16, "packed psi"))) return rc;
   if ((rc = alloc_bytes(&S->d_mid, S->mid_len * 16, "packed intermediates"))) return rc;
+  // padding rows of intermediate panels are never written; keep them zero
+  if (S->mid_len > 0) TSC_HIP_CHECK(hipMemsetAsync(S->d_mid, 0, (size_t)S->mid_len * 16, st));
   std::vector<PkJob> jobs_ops = make_jobs(pan[0]), jobs_in = make_jobs(pan[1]);
   S->n_ops_jobs = (int)jobs_ops.size();
   S->n_in_jobs = (int)jobs_in.size();
@@ -651,7 +739,8 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
   int64_t slots = 0;
   for (int s = 0; s < 2; ++s) {
     PkStage &G = S->st[s];
-    Units un = make_tiles(cps[s].data(), p->h_o[s].data(), (int32_t)p->h_o[s].size());
+    // 128x128 units, 4 waves each; two workgroups per CU
+    Units un = make_tiles(cps[s].data(), p->h_o[s].data(), (int32_t)p->h_o[s].size(), WT, WT, 2048.0);
     G.nt = (int)un.tiles.size();
     G.nr = (int)un.reds.size();
     G.nslot = un.nslot;
@@ -662,7 +751,7 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
     if ((rc = upload_vec(&G.d_r, un.reds, st))) return rc;
   }
   if ((rc = upload_vec(&S->st[0].d_po, pouts, st))) return rc;
-  if ((rc = alloc_bytes(&S->d_part, slots * (int64_t)TM * TN * (int64_t)sizeof(float2), "partial tiles"))) return rc;
+  if ((rc = alloc_bytes(&S->d_part, slots * (int64_t)WT * WT * (int64_t)sizeof(float2), "partial tiles"))) return rc;
   if ((rc = launch_pack(p->d_ops64, S->d_ops, S->d_ops_panels, S->d_ops_jobs, S->n_ops_jobs, st))) return rc;
   TSC_HIP_CHECK(hipStreamSynchronize(st));
   return TSC_OK;
@@ -709,7 +798,7 @@ int pk_enqueue(tsc_plan *p, const double2 *in, double2 *out, bool events) {
   for (int g = 0; g < 2; ++g) {
     const PkStage &G = S->st[g];
     if (G.nt > 0) {
-      hipLaunchKernelGGL(pk_gemm_kernel, dim3(G.nt), dim3(64), 0, s, P, reinterpret_cast<bf8 *>(S->d_mid), in,
+      hipLaunchKernelGGL(pk_gemm_kernel, dim3(G.nt), dim3(256), 0, s, P, reinterpret_cast<bf8 *>(S->d_mid), in,
                          p->d_ops64, out, p->d_o[g], S->st[0].d_po, G.d_cp, G.d_pc, G.d_t,
                          reinterpret_cast<float2 *>(S->d_part));
       TSC_HIP_CHECK(hipGetLastError());
