@@ -33,8 +33,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="yc6-m6000", choices=["yc6-m6000", "synthetic400", "synthetic-small"])
-    ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-lanczos", action="store_true", help="H.psi only (no Lanczos, truncation or DMRG check)")
     ap.add_argument("--lanczos-iters", type=int, default=40)
+    ap.add_argument("--refine", type=int, default=8, help="FP64 Lanczos steps after the mixed ones")
+    ap.add_argument("--no-dmrg", action="store_true", help="skip the FP64-vs-mixed DMRG energy check")
+    ap.add_argument("--dmrg-sweeps", type=int, default=6)
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "f32", "f64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -144,11 +147,15 @@ def time_gpu(ctx, plan, vin, vout, prec, steps, warmup):
 
 
 def lanczos_and_step(ctx, plan, wl, vin, prec, args):
-    """FP64 vs mixed Lanczos energies per site on the same superblock and the
-    device time of one DMRG step's hot path (Lanczos, rho/eigensolve/
-    truncation); sweep time extrapolated from it."""
+    """Device time of one DMRG step's hot path on the centre superblock in
+    FP64 and in mixed precision: Lanczos at a fixed Krylov budget (FP64: `it`
+    H.psi; mixed: `it` mixed H.psi then `refine` FP64 H.psi restarted from the
+    mixed Ritz vector), then rho / eigensolve / truncation. The sweep time is
+    extrapolated from it. The blocks are random, so these Lanczos energies
+    are not converged and their difference measures convergence, not
+    precision; the precision check is dmrg_precision()."""
     from tsc_tri_t1t2j1j2_amd import capi
-    it = args.lanczos_iters
+    it, refine = args.lanczos_iters, args.refine
     psi64 = ctx.vec(wl.psi_len)
     psi64.copy_from(vin)
     ctx.sync()
@@ -159,7 +166,7 @@ def lanczos_and_step(ctx, plan, wl, vin, prec, args):
     psim.copy_from(vin)
     ctx.sync()
     t0 = time.perf_counter()
-    em, (mvm, _) = plan.lanczos(psim, tol=1e-13, max_inner=it, max_outer=0, prec=prec, refine_iters=8)
+    em, (mvm, _) = plan.lanczos(psim, tol=1e-13, max_inner=it, max_outer=0, prec=prec, refine_iters=refine)
     tm = time.perf_counter() - t0
     print(f"# lanczos f64 {t64:.2f}s ({mv64} H.psi) E={e64:.12f}; mixed {tm:.2f}s ({mvm} H.psi) E={em:.12f}",
           file=sys.stderr, flush=True)
@@ -171,15 +178,36 @@ def lanczos_and_step(ctx, plan, wl, vin, prec, args):
     t0 = time.perf_counter()
     tr = capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, 6000, 1e-15)
     ttr = time.perf_counter() - t0
-    step = tm + ttr
     nsteps = 2 * (wl.N - 3)
-    return {"lanczos_iters": it, "E_per_site_f64": e64 / wl.N, "E_per_site_mixed_refined": em / wl.N,
-            "dE_per_site_mixed_vs_f64": abs(em - e64) / wl.N, "lanczos_f64_s": round(t64, 3),
-            "lanczos_mixed_s": round(tm, 3), "rho_truncate_s": round(ttr, 3), "kept": int(tr["num_state"]),
-            "step_s_mixed": round(step, 3),
-            "sweep_s_estimate": round(step * nsteps, 1),
-            "sweep_estimate_note": f"{nsteps} centre-sized steps x (Lanczos {it} it + rho/truncation); "
-                                   "plan build and renormalisation excluded"}
+    s64, smx = t64 + ttr, tm + ttr
+    return {"lanczos_budget": f"FP64 {it} H.psi; mixed {it} + {refine} FP64 refinement H.psi",
+            "lanczos_f64_s": round(t64, 3), "lanczos_mixed_s": round(tm, 3), "rho_truncate_s": round(ttr, 3),
+            "kept": int(tr["num_state"]), "step_s_f64": round(s64, 3), "step_s_mixed": round(smx, 3),
+            "sweep_s_f64_estimate": round(s64 * nsteps, 1), "sweep_s_mixed_estimate": round(smx * nsteps, 1),
+            "sweep_speedup_mixed_vs_f64": round(s64 / smx, 3),
+            "sweep_estimate_note": f"{nsteps} centre-sized steps x (Lanczos + rho/truncation); plan build, "
+                                   "renormalisation and the wave-function guess excluded"}
+
+
+def dmrg_precision(ctx, sweeps, refine):
+    """|dE|/site between an FP64 and a mixed (BF16x3 + `refine` FP64 Lanczos
+    steps per solve) run of the full DMRG on YC4 4x6 (24 sites, 22 electrons,
+    kept_max 256)."""
+    from tsc_tri_t1t2j1j2_amd import capi
+    from tsc_tri_t1t2j1j2_amd.dmrg import DMRG
+    out = {}
+    for name, prec, rf in (("f64", capi.PREC_F64, 0), ("bf16x3", capi.PREC_BF16X3, refine)):
+        t0 = time.perf_counter()
+        d = DMRG(ctx, 6, 4, kept_min=110, kept_max=256, sweeps=sweeps, lanzero=1e-10, prec=prec, refine=rf,
+                 seed=1234)
+        E = d.run()
+        out[name] = (E, time.perf_counter() - t0, d.N)
+        print(f"# dmrg YC4 6x4 M=256 {name}: E={E:.12f} in {out[name][1]:.1f}s", file=sys.stderr, flush=True)
+    N = out["f64"][2]
+    return {"dmrg_check": f"YC4 4x6 (24 sites, 22 electrons) kept_max 256, {sweeps} sweeps, lanzero 1e-10",
+            "dmrg_E_per_site_f64": out["f64"][0] / N, "dmrg_E_per_site_mixed": out["bf16x3"][0] / N,
+            "dmrg_dE_per_site_mixed_vs_f64": abs(out["f64"][0] - out["bf16x3"][0]) / N, "dmrg_refine": refine,
+            "dmrg_wall_s_f64": round(out["f64"][1], 2), "dmrg_wall_s_mixed": round(out["bf16x3"][1], 2)}
 
 
 def cpu_baseline(sb, psi_host, seconds):
@@ -273,6 +301,8 @@ def main():
     extra = {}
     if isinstance(sb, PhysicsWorkload) and not args.no_lanczos:
         extra = lanczos_and_step(ctx, plan, sb, vin, prec, args)
+    if isinstance(sb, PhysicsWorkload) and not (args.no_lanczos or args.no_dmrg) and prec != capi.PREC_F64:
+        extra.update(dmrg_precision(ctx, args.dmrg_sweeps, args.refine))
     cpu = None
     if not args.no_cpu:
         psi_host = vin.download()

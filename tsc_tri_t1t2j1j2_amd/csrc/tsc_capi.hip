@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 
 #include "tsc_internal.h"
@@ -252,10 +253,13 @@ namespace tsc {
 //
 // Placement: the device hands workgroup b to XCD b % kXcds, and each XCD
 // has its own L2. All units of one output block read the same operand
-// windows, so they form one group that goes to a single XCD, units of the
-// same contribution chunk next to each other; groups are dealt to the XCDs
-// largest first onto the least-loaded one, and the per-XCD lists are
-// interleaved (padded with empty units, blk = -1, that exit at once).
+// windows, and output blocks whose smallest B operand window is the same
+// (blocks of one environment window: they read the same operator blocks)
+// do too; such blocks form one group that goes to a single XCD, units of
+// the same contribution chunk next to each other. Groups (capped at a
+// quarter of an XCD's share of the work) are dealt to the XCDs largest
+// first onto the least-loaded one, and the per-XCD lists are interleaved
+// (padded with empty units, blk = -1, that exit at once).
 constexpr int kXcds = 8;  // MI355X: 256 CUs in 8 XCDs
 
 Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm, int tn, double desired) {
@@ -280,11 +284,17 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm
     std::vector<Tile> units;
   };
   std::vector<Group> groups;
+  std::map<std::pair<int, int64_t>, size_t> open_group;  // smallest B window -> group being filled
+  const double group_cap = total / (4.0 * kXcds);
   for (int32_t b = 0; b < no; ++b) {
     const tsc_outblock &ob = o[b];
     if (ob.m == 0 || ob.n == 0) continue;
     double w = 0.0;
-    for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) w += ccost(ci);
+    std::pair<int, int64_t> key(1 << 30, 0);
+    for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) {
+      w += ccost(ci);
+      if (c[ci].a_op != TSC_OP_I && c[ci].b_op != TSC_OP_I) key = std::min(key, std::make_pair((int)c[ci].b_buf, c[ci].b_off));
+    }
     const int nti = (ob.m + tm - 1) / tm, ntj = (ob.n + tn - 1) / tn;
     std::vector<int32_t> cut{ob.cbeg};
     std::vector<double> ccut;
@@ -312,7 +322,13 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm
                                   nch});
       u.nslot += (int64_t)nti * ntj * nch;
     }
-    Group g;
+    auto it = open_group.find(key);
+    if (it == open_group.end() || groups[it->second].cost >= group_cap) {
+      groups.emplace_back();
+      open_group[key] = groups.size() - 1;
+      it = open_group.find(key);
+    }
+    Group &g = groups[it->second];
     for (int ch = 0; ch < nch; ++ch)
       for (int ti = 0; ti < nti; ++ti)
         for (int tj = 0; tj < ntj; ++tj) {
@@ -320,7 +336,6 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm
           g.units.push_back(Tile{b, (int16_t)ti, (int16_t)tj, cut[ch], cut[ch + 1], slot, 0});
           g.cost += ccut[ch];
         }
-    groups.push_back(std::move(g));
   }
   std::stable_sort(groups.begin(), groups.end(), [](const Group &x, const Group &y) { return x.cost > y.cost; });
   std::vector<std::vector<Tile>> lists(kXcds);
@@ -372,6 +387,7 @@ extern "C" int tsc_plan_destroy(tsc_plan *p) {
   hipFree(p->d_part64);
   hipFree(p->d_part32);
   tsc::pk_free(p->pk);
+  for (auto v : p->kry) hipFree(v);
   for (int i = 0; i < 3; ++i) {
     if (p->ev[i]) hipEventDestroy(p->ev[i]);
     if (p->gexec[i]) hipGraphExecDestroy(p->gexec[i]);
@@ -650,12 +666,20 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
   const int64_t n = psi->n;
   TSC_REQUIRE(max_inner >= 1 && max_inner + 1 <= kMaxVecs, "max_inner out of range");
   TSC_REQUIRE(n == p->in_elems && n == p->out_elems, "psi size does not match the plan");
-  std::vector<double2 *> V(max_inner + 1, nullptr);
-  for (auto &v : V) TSC_HIP_CHECK(hipMalloc((void **)&v, sizeof(double2) * std::max<int64_t>(n, 1)));
-  auto free_all = [&]() {
+  // Krylov vectors stay allocated with the plan between calls
+  if (p->kry_n != n) {
     hipStreamSynchronize(ctx->stream);
-    for (auto v : V) hipFree(v);
-  };
+    for (auto v : p->kry) hipFree(v);
+    p->kry.clear();
+    p->kry_n = n;
+  }
+  while ((int)p->kry.size() < max_inner + 1) {
+    double2 *v = nullptr;
+    TSC_HIP_CHECK(hipMalloc((void **)&v, sizeof(double2) * std::max<int64_t>(n, 1)));
+    p->kry.push_back(v);
+  }
+  std::vector<double2 *> V(p->kry.begin(), p->kry.begin() + max_inner + 1);
+  auto free_all = [&]() { hipStreamSynchronize(ctx->stream); };
   std::vector<double> alpha(max_inner + 1, 0.0), beta(max_inner + 1, 0.0), z(max_inner + 1), uu(max_inner + 1, 1.0);
   double lastmin = 1024.0;
   int matvecs = 0, iter_out = 0, last = 1;

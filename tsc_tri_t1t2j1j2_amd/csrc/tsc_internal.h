@@ -199,6 +199,9 @@ struct tsc_plan {
   std::vector<tsc_contrib> h_c[2];
   std::vector<tsc_outblock> h_o[2];
   tsc::PkState *pk = nullptr;
+  // Lanczos Krylov vectors (kry_n elements each), kept between calls
+  std::vector<double2 *> kry;
+  int64_t kry_n = 0;
 };
 
 namespace tsc {

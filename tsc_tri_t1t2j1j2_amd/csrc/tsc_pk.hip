@@ -180,6 +180,7 @@ __global__ __launch_bounds__(256, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restri
   const int i0 = t.ti * WT, j0 = t.tj * WT;
   const int iw = i0 + 64 * wa, jw = j0 + 64 * wb;
   const bool active = iw < m && jw < n;
+  const bool a1 = iw + 32 < m;  // second 32-row block of this wave's quarter holds rows
   // loader role of this wave
   const bool ldA = wave < 2;
   const int lrb = wave & 1;
@@ -246,36 +247,44 @@ __global__ __launch_bounds__(256, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restri
     }
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
+      if (jw + 32 * b >= n) continue;  // columns past the block: nothing to compute
       const bf8 nbh = neg8(B[b][1]), nbl = neg8(B[b][3]);
       // planes: 0 re hi, 1 im hi, 2 re lo, 3 im lo
       // Re += br*ar - bi*ai ; Im += bi*ar + br*ai (each product in three terms)
+      // (the second 32-row block only when it holds rows of the output block)
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !a1) break;
         accR[a][b] = MFMA32(B[b][0], A[a][0], accR[a][b]);
         accI[a][b] = MFMA32(B[b][1], A[a][0], accI[a][b]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !a1) break;
         accR[a][b] = MFMA32(B[b][2], A[a][0], accR[a][b]);
         accI[a][b] = MFMA32(B[b][3], A[a][0], accI[a][b]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !a1) break;
         accR[a][b] = MFMA32(B[b][0], A[a][2], accR[a][b]);
         accI[a][b] = MFMA32(B[b][1], A[a][2], accI[a][b]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !a1) break;
         accR[a][b] = MFMA32(nbh, A[a][1], accR[a][b]);
         accI[a][b] = MFMA32(B[b][0], A[a][1], accI[a][b]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !a1) break;
         accR[a][b] = MFMA32(nbl, A[a][1], accR[a][b]);
         accI[a][b] = MFMA32(B[b][2], A[a][1], accI[a][b]);
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
+        if (a == 1 && !a1) break;
         accR[a][b] = MFMA32(nbh, A[a][3], accR[a][b]);
         accI[a][b] = MFMA32(B[b][0], A[a][3], accI[a][b]);
       }
@@ -615,6 +624,7 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
         if (c.a_op == TSC_OP_I || c.b_op == TSC_OP_I) return 1;
         return (c.coef_re == 0.0 && c.coef_im == 0.0) ? 2 : 0;
       };
+      // GEMM terms ordered by coefficient (few accumulator rescales)
       std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         const int cx = cls(x), cy = cls(y);
         if (cx != cy) return cx < cy;

@@ -25,23 +25,6 @@ struct DevBuf {
   }
 };
 
-int ensure_blas(tsc_ctx *ctx, rocblas_handle *h) {
-  static thread_local rocblas_handle handle = nullptr;
-  static thread_local int handle_dev = -1;
-  if (!handle || handle_dev != ctx->device) {
-    if (handle) rocblas_destroy_handle(handle);
-    if (rocblas_create_handle(&handle) != rocblas_status_success) {
-      set_error("rocblas_create_handle failed");
-      handle = nullptr;
-      return TSC_EHIP;
-    }
-    handle_dev = ctx->device;
-  }
-  rocblas_set_stream(handle, ctx->stream);
-  *h = handle;
-  return TSC_OK;
-}
-
 // rho_s for every sector, into d_rho (sum d_s^2), through a stage-2-only plan
 int build_rho(tsc_ctx *ctx, const double2 *d_psi, int64_t psi_elems, int32_t nsec, const int32_t *sys_dim,
               const int32_t *env_dim, int side, double2 *d_rho, int64_t rho_elems, std::vector<int64_t> &rho_off) {
@@ -102,30 +85,189 @@ int build_rho(tsc_ctx *ctx, const double2 *d_psi, int64_t psi_elems, int32_t nse
   return rc;
 }
 
-// in-place ascending eigensolve of every rho_s (ZHEEV 'V','U' role,
-// den_mat.f90:229-253); eigenvalues to d_w (sum d_s)
-int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, const std::vector<int64_t> &rho_off,
-                double *d_w, const std::vector<int64_t> &w_off) {
-  rocblas_handle h;
-  int rc = ensure_blas(ctx, &h);
-  if (rc) return rc;
-  int maxd = 1;
-  for (int32_t s = 0; s < nsec; ++s) maxd = std::max(maxd, dim[s]);
-  DevBuf e, info;
-  TSC_HIP_CHECK(hipMalloc(&e.p, sizeof(double) * maxd));
-  TSC_HIP_CHECK(hipMalloc(&info.p, sizeof(rocblas_int)));
-  for (int32_t s = 0; s < nsec; ++s) {
-    const int d = dim[s];
-    if (d == 0) continue;
-    rocblas_status st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, d,
-                                         reinterpret_cast<rocblas_double_complex *>(d_rho + rho_off[s]), d,
-                                         d_w + w_off[s], (double *)e.p, (rocblas_int *)info.p);
-    if (st != rocblas_status_success) {
-      set_error("rocsolver_zheevd failed for sector " + std::to_string(s));
+// Batched eigensolve. Sectors are sorted by dimension; the small ones are
+// cut into buckets of similar size, each one batched Jacobi call
+// (rocsolver_zheevj_strided_batched) on copies padded to the bucket's
+// largest dimension P with -1 on the padding diagonal ([[rho, 0], [0, -I]]:
+// rho's eigenpairs are untouched and its eigenvalues, >= 0, sort after the
+// -1s, so they are the last d of P); each large sector is a plain zheevd.
+struct EigSlot {
+  int64_t rho_off;   // rho_s in the packed rho array (d x d, column-major)
+  int64_t slot_off;  // padded copy in the bucket buffer (P x P)
+  int64_t w_off;     // eigenvalues of sector s in the output array
+  int32_t d, pad;
+};
+
+__global__ void eig_pad_kernel(const double2 *__restrict__ rho, double2 *__restrict__ buf,
+                               const EigSlot *__restrict__ slots, int P) {
+  const EigSlot sl = slots[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)P * P) return;
+  const int i = (int)(e % P), j = (int)(e / P);
+  double2 v = make_double2(i == j ? -1.0 : 0.0, 0.0);
+  if (i < sl.d && j < sl.d) v = rho[sl.rho_off + i + (int64_t)j * sl.d];
+  buf[sl.slot_off + e] = v;
+}
+
+__global__ void eig_unpad_kernel(double2 *__restrict__ rho, const double2 *__restrict__ buf,
+                                 const double *__restrict__ wbuf, double *__restrict__ w,
+                                 const EigSlot *__restrict__ slots, int P) {
+  const EigSlot sl = slots[blockIdx.y];
+  const int d = sl.d;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (int64_t)d * d) {
+    const int i = (int)(e % d), j = (int)(e / d);
+    rho[sl.rho_off + e] = buf[sl.slot_off + i + (int64_t)(P - d + j) * P];
+  }
+  if (e < d) w[sl.w_off + e] = wbuf[(int64_t)blockIdx.y * P + P - d + e];
+}
+
+constexpr int kJacobiMax = 64;  // largest sector solved in a padded Jacobi batch
+constexpr int kEigStreams = 8;  // side streams for concurrent eigensolves
+
+// Side streams (default priority) with one rocBLAS handle each, per device
+// and host thread, kept for the life of the process.
+struct EigLanes {
+  int device = -1;
+  hipStream_t stream[kEigStreams] = {};
+  rocblas_handle handle[kEigStreams] = {};
+  hipEvent_t done[kEigStreams] = {};
+  hipEvent_t start = nullptr;
+};
+
+int eig_lanes(tsc_ctx *ctx, EigLanes **out) {
+  static thread_local std::vector<EigLanes> lanes;
+  for (auto &l : lanes)
+    if (l.device == ctx->device) {
+      *out = &l;
+      return TSC_OK;
+    }
+  EigLanes l;
+  l.device = ctx->device;
+  for (int i = 0; i < kEigStreams; ++i) {
+    TSC_HIP_CHECK(hipStreamCreate(&l.stream[i]));
+    TSC_HIP_CHECK(hipEventCreateWithFlags(&l.done[i], hipEventDisableTiming));
+    if (rocblas_create_handle(&l.handle[i]) != rocblas_status_success) {
+      set_error("rocblas_create_handle failed");
       return TSC_EHIP;
     }
+    rocblas_set_stream(l.handle[i], l.stream[i]);
   }
-  TSC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  TSC_HIP_CHECK(hipEventCreateWithFlags(&l.start, hipEventDisableTiming));
+  lanes.push_back(l);
+  *out = &lanes.back();
+  return TSC_OK;
+}
+
+// ascending eigensolve of every rho_s (ZHEEV 'V','U' role, den_mat.f90:229-253):
+// eigenvectors replace rho_s in place, eigenvalues go to d_w. Buckets run
+// concurrently on the side streams (the solver is launch-bound at these
+// sizes), largest first onto the least-loaded stream.
+int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, const std::vector<int64_t> &rho_off,
+                double *d_w, const std::vector<int64_t> &w_off) {
+  EigLanes *L = nullptr;
+  int rc = eig_lanes(ctx, &L);
+  if (rc) return rc;
+  std::vector<int32_t> order;
+  for (int32_t s = 0; s < nsec; ++s)
+    if (dim[s] > 0) order.push_back(s);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return dim[x] < dim[y]; });
+  struct Bucket {
+    size_t b0, cnt;
+    int P;
+    int64_t buf_off, w_off, slot0;  // into the shared work buffers
+  };
+  std::vector<Bucket> buckets;
+  std::vector<EigSlot> slots;
+  int64_t buf_len = 0, wlen = 0;
+  for (size_t b0 = 0; b0 < order.size();) {
+    // bucket: sectors of dimension <= kJacobiMax within 25% (+8) of its
+    // smallest; larger sectors one by one
+    const int dmin = dim[order[b0]];
+    size_t b1 = b0 + 1;
+    if (dmin <= kJacobiMax)
+      while (b1 < order.size() && dim[order[b1]] <= std::min(kJacobiMax, dmin + dmin / 4 + 8)) ++b1;
+    const int P = dim[order[b1 - 1]];
+    Bucket bk{b0, b1 - b0, P, buf_len, wlen, (int64_t)slots.size()};
+    for (size_t q = 0; q < bk.cnt; ++q) {
+      const int32_t s = order[b0 + q];
+      slots.push_back(EigSlot{rho_off[s], buf_len + (int64_t)q * P * P, w_off[s], dim[s], 0});
+    }
+    buf_len += (int64_t)bk.cnt * P * P;
+    wlen += (int64_t)bk.cnt * P;
+    buckets.push_back(bk);
+    b0 = b1;
+  }
+  if (buckets.empty()) return TSC_OK;
+  const int64_t nslots = (int64_t)slots.size();
+  DevBuf buf, wbuf, ebuf, info, dsl, resid, sweeps;
+  TSC_HIP_CHECK(hipMalloc(&buf.p, sizeof(double2) * (size_t)buf_len));
+  TSC_HIP_CHECK(hipMalloc(&wbuf.p, sizeof(double) * (size_t)wlen));
+  TSC_HIP_CHECK(hipMalloc(&ebuf.p, sizeof(double) * (size_t)wlen));
+  TSC_HIP_CHECK(hipMalloc(&info.p, sizeof(rocblas_int) * nslots));
+  TSC_HIP_CHECK(hipMalloc(&resid.p, sizeof(double) * nslots));
+  TSC_HIP_CHECK(hipMalloc(&sweeps.p, sizeof(rocblas_int) * nslots));
+  TSC_HIP_CHECK(hipMalloc(&dsl.p, sizeof(EigSlot) * nslots));
+  hipStream_t st = ctx->stream;
+  TSC_HIP_CHECK(hipMemcpyAsync(dsl.p, slots.data(), sizeof(EigSlot) * nslots, hipMemcpyHostToDevice, st));
+  TSC_HIP_CHECK(hipEventRecord(L->start, st));
+  for (int i = 0; i < kEigStreams; ++i) TSC_HIP_CHECK(hipStreamWaitEvent(L->stream[i], L->start, 0));
+  // largest buckets first, each onto the least-loaded side stream
+  std::vector<size_t> border(buckets.size());
+  std::iota(border.begin(), border.end(), 0);
+  auto work = [&](const Bucket &bk) { return (double)bk.cnt * bk.P * bk.P * bk.P; };
+  std::stable_sort(border.begin(), border.end(), [&](size_t x, size_t y) { return work(buckets[x]) > work(buckets[y]); });
+  std::vector<double> load(kEigStreams, 0.0);
+  for (size_t bi : border) {
+    const Bucket &bk = buckets[bi];
+    const int lane = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[lane] += work(bk) + 1e6;  // fixed cost per call
+    hipStream_t ls = L->stream[lane];
+    rocblas_handle h = L->handle[lane];
+    const int P = bk.P, cnt = (int)bk.cnt;
+    const int64_t pp = (int64_t)P * P;
+    const EigSlot *dslot = (const EigSlot *)dsl.p + bk.slot0;
+    double2 *bbuf = (double2 *)buf.p + bk.buf_off;
+    double *bw = (double *)wbuf.p + bk.w_off;
+    rocblas_int *binfo = (rocblas_int *)info.p + bk.slot0;
+    dim3 grid((unsigned)((pp + 255) / 256), (unsigned)cnt);
+    // slot offsets address the shared buffer; pass its base
+    hipLaunchKernelGGL(eig_pad_kernel, grid, dim3(256), 0, ls, (const double2 *)d_rho, (double2 *)buf.p, dslot, P);
+    TSC_HIP_CHECK(hipGetLastError());
+    rocblas_status rs;
+    if (P > kJacobiMax) {
+      // a large sector alone: no padding (P == d), divide and conquer
+      rs = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, P,
+                            reinterpret_cast<rocblas_double_complex *>(bbuf), P, bw, (double *)ebuf.p + bk.w_off,
+                            binfo);
+    } else {
+      // padded batch: Jacobi (exactly decoupled, degenerate padding is harmless to it)
+      rs = rocsolver_zheevj_strided_batched(h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper,
+                                            P, reinterpret_cast<rocblas_double_complex *>(bbuf), P, pp, 0.0,
+                                            (double *)resid.p + bk.slot0, 100, (rocblas_int *)sweeps.p + bk.slot0,
+                                            bw, P, binfo, cnt);
+    }
+    if (rs != rocblas_status_success) {
+      set_error("rocSOLVER Hermitian eigensolver failed (n = " + std::to_string(P) + ", batch " +
+                std::to_string(cnt) + ")");
+      return TSC_EHIP;
+    }
+    hipLaunchKernelGGL(eig_unpad_kernel, grid, dim3(256), 0, ls, d_rho, (const double2 *)buf.p, bw, d_w, dslot, P);
+    TSC_HIP_CHECK(hipGetLastError());
+  }
+  for (int i = 0; i < kEigStreams; ++i) {
+    TSC_HIP_CHECK(hipEventRecord(L->done[i], L->stream[i]));
+    TSC_HIP_CHECK(hipStreamWaitEvent(st, L->done[i], 0));
+  }
+  std::vector<rocblas_int> hinfo(nslots);
+  TSC_HIP_CHECK(hipMemcpyAsync(hinfo.data(), info.p, sizeof(rocblas_int) * nslots, hipMemcpyDeviceToHost, st));
+  TSC_HIP_CHECK(hipStreamSynchronize(st));
+  for (const Bucket &bk : buckets)
+    for (size_t q = 0; q < bk.cnt; ++q)
+      if (hinfo[bk.slot0 + q] != 0) {
+        set_error("Hermitian eigensolver did not converge for sector " + std::to_string(order[bk.b0 + q]));
+        return TSC_EHIP;
+      }
   return TSC_OK;
 }
 
@@ -234,6 +376,11 @@ int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_
     uoff += (int64_t)d * k;
   }
   TSC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < 2 * uoff; ++i)
+    if (std::isnan(u_out[i])) {
+      set_error("density-matrix eigensolver returned NaN eigenvectors");
+      return TSC_EHIP;
+    }
   return TSC_OK;
 }
 
