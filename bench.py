@@ -27,6 +27,19 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0}
 HBM_GBS = 8000.0
 
 
+# HBM bytes per launch of the dominant kernel (stage 2 of H.psi), YC6 M=6000
+# centre superblock, from two rocprofv3 runs of
+#   bench.py --steps 2 --warmup 1 --no-cpu --no-lanczos --calibrate-copy-mib 1024
+# with --pmc FETCH_SIZE and --pmc WRITE_SIZE; each counter is scaled by its
+# reading for the 1 GiB copy in the same run (FETCH_SIZE x 2.0, WRITE_SIZE
+# x 1.0). Per-dispatch KiB: BF16x3 pk_gemm_kernel 14388044 fetched, 617783
+# written; FP64 grouped_kernel 29900490 fetched, 344647 written.
+MEASURED_TRAFFIC = {
+    ("yc6-m6000", "bf16x3"): (14388044 * 2.0 + 617783 * 1.0) * 1024,
+    ("yc6-m6000", "f64"): (29900490 * 2.0 + 344647 * 1.0) * 1024,
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,8 +307,9 @@ def main():
         dom, kms, kf = "grouped stage 2", k2, f2
     achieved = kf / (kms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.prec]
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.prec)) if dom == "grouped stage 2" else None
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 5), "traffic": None, "kernel": dom,
+            "frac": round(achieved / peak, 5), "traffic": traffic, "kernel": dom,
             "kernel_ms": round(kms, 3), "flops_per_launch": kf}
 
     extra = {}
