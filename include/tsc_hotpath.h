@@ -124,6 +124,16 @@ int tsc_vec_fill_gaussian(tsc_vec *v, uint64_t seed, uint64_t key, double sigma)
 int tsc_vec_axpby(tsc_vec *y, double a, const tsc_vec *x, double b);
 /* y = x (same length) */
 int tsc_vec_copy(tsc_vec *y, const tsc_vec *x);
+/*
+ * Element ranges, asynchronous on the context stream (host arrays must stay
+ * valid until tsc_ctx_sync): y[y_off, y_off+n) = x[x_off, x_off+n) on the
+ * device; v[off, off+n) = host; host = v[off, off+n). Block operators live
+ * in device arenas between DMRG steps (the reference keeps them in
+ * Total_Model%sub(:)%mat, common.f90:84-174); these move sub-ranges of them.
+ */
+int tsc_vec_copy_range(tsc_vec *y, int64_t y_off, const tsc_vec *x, int64_t x_off, int64_t n);
+int tsc_vec_upload_range(tsc_vec *v, int64_t off, const double *host_c128, int64_t n);
+int tsc_vec_download_range(const tsc_vec *v, int64_t off, double *host_c128, int64_t n);
 
 /*
  * ---- grouped two-stage contraction plan (the H.psi engine) --------------
@@ -138,6 +148,17 @@ int tsc_plan_create(tsc_ctx *ctx, const double *ops_host_c128, int64_t ops_elems
                     const tsc_outblock *o1, int32_t no1, const tsc_contrib *c2,
                     int32_t nc2, const tsc_outblock *o2, int32_t no2,
                     int64_t in_elems, int64_t out_elems, tsc_plan **out);
+/*
+ * Same, with the operator arena borrowed from a device vector (ops->n
+ * elements, not copied; it must outlive the plan). The plan's output vector
+ * may be that same vector when the output windows do not overlap the
+ * operator blocks read (used to merge operators in place, as hmerge does,
+ * get_sys.f90:7398-7626).
+ */
+int tsc_plan_create_dev(tsc_ctx *ctx, const tsc_vec *ops, int64_t mid_elems, const tsc_contrib *c1,
+                        int32_t nc1, const tsc_outblock *o1, int32_t no1, const tsc_contrib *c2,
+                        int32_t nc2, const tsc_outblock *o2, int32_t no2, int64_t in_elems,
+                        int64_t out_elems, tsc_plan **out);
 int tsc_plan_destroy(tsc_plan *p);
 /* replace the operator arena values (same layout) */
 int tsc_plan_update_ops(tsc_plan *p, const double *ops_host_c128);
@@ -292,6 +313,8 @@ int tsc_tasks_copy(const tsc_tasks *t, tsc_contrib *c1, tsc_outblock *o1, tsc_co
                    tsc_outblock *o2, int32_t *super_xy, int32_t *lsec, int32_t *rsec);
 int tsc_tasks_plan(tsc_ctx *ctx, const tsc_tasks *t, const double *arena_c128, int64_t arena_len,
                    tsc_plan **plan);
+/* the same plan over an operator arena already on the device (borrowed) */
+int tsc_tasks_plan_dev(tsc_ctx *ctx, const tsc_tasks *t, const tsc_vec *arena, tsc_plan **plan);
 int tsc_tasks_destroy(tsc_tasks *t);
 
 /* Racah 6j symbol with doubled arguments (W6JS, block.f90:3744-3795) */

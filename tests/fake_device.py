@@ -17,6 +17,19 @@ class FakeVec:
     def download(self):
         return self.x.copy()
 
+    def upload(self, x):
+        self.x[:] = np.asarray(x, complex).ravel(order="K")
+
+    def copy_range(self, off, src, src_off, n):
+        self.x[off:off + n] = src.x[src_off:src_off + n]
+
+    def upload_range(self, off, x):
+        x = np.asarray(x, complex).ravel(order="K")
+        self.x[off:off + x.size] = x
+
+    def download_range(self, off, n):
+        return self.x[off:off + n].copy()
+
     def close(self):
         pass
 
@@ -25,19 +38,43 @@ class FakeCtx:
     def vec_from(self, x):
         return FakeVec(np.asarray(x).ravel(order="F"))
 
+    def vec(self, n):
+        return FakeVec(np.zeros(int(n), complex))
+
+    def sync(self):
+        pass
+
 
 class FakePlan:
     def __init__(self, ctx, ops, ops_elems, mid_elems, c1, o1, c2, o2, in_elems, out_elems):
         self.args = (np.asarray(ops, complex) if ops is not None else np.zeros(max(ops_elems, 1), complex),
                      c1, o1, c2, o2, mid_elems)
         self.in_elems, self.out_elems = in_elems, out_elems
+        self.ops_vec = None
+
+    @classmethod
+    def create_dev(cls, ctx, ops_vec, mid_elems, c1, o1, c2, o2, in_elems, out_elems):
+        pl = cls(ctx, None, 1, mid_elems, c1, o1, c2, o2, in_elems, out_elems)
+        pl.ops_vec = ops_vec   # read at each application, as the device borrows it
+        return pl
 
     def _mv(self, x):
         ops, c1, o1, c2, o2, mid = self.args
+        if self.ops_vec is not None:
+            ops = self.ops_vec.x
         return plan_exec.execute(ops, c1, o1, c2, o2, mid, x, self.out_elems)
 
     def apply_host(self, x, prec=0):
         return self._mv(np.asarray(x, complex))
+
+    def apply(self, vin, vout, prec=0):
+        """Writes the output windows only; the rest of vout is untouched."""
+        res = self._mv(vin.x)
+        o2 = self.args[4]
+        for ob in o2:
+            ldc, m, n, c = int(ob["ldc"]), int(ob["m"]), int(ob["n"]), int(ob["c_off"])
+            for j in range(n):
+                vout.x[c + j * ldc:c + j * ldc + m] = res[c + j * ldc:c + j * ldc + m]
 
     def lanczos(self, psi, tol=1e-10, max_inner=100, max_outer=4, prec=0, refine_iters=0):
         e, v, nmv = olanczos.lanczos(self._mv, psi.x, tol=tol, max_inner=max_inner, max_outer=max_outer)

@@ -8,16 +8,30 @@ from tests import fake_device
 from tsc_tri_t1t2j1j2_amd.dmrg import DMRG
 
 
-@pytest.mark.parametrize("kmax", [4000, 40])
-def test_driver_on_fake_device(monkeypatch, kmax):
+@pytest.mark.parametrize("kmax,resident", [(4000, False), (40, False), (4000, True), (40, True)])
+def test_driver_on_fake_device(monkeypatch, kmax, resident):
     ctx = fake_device.install(monkeypatch)
     Lx, Ny, Ne = 3, 3, 8
     b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
     e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
     log = []
     d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=min(20, kmax), kept_max=kmax, sweeps=2, lanzero=1e-10, trun_err=1e-12,
-             log=log.append, native_planner=False)
+             log=log.append, native_planner=False, resident=resident)
     e = d.run()
     print("\n".join(log))
     assert e >= e_ed - 1e-9
     assert (e - e_ed) / (Lx * Ny) < (1e-9 if kmax > 1000 else 1e-4), (e, e_ed)
+
+
+def test_resident_driver_matches_host_driver(monkeypatch):
+    """Operators kept in (fake) device arenas, merged and renormalised there,
+    give the same DMRG trajectory as host-side blocks."""
+    ctx = fake_device.install(monkeypatch)
+    runs = []
+    for resident in (False, True):
+        d = DMRG(ctx, 3, 3, Ne=8, kept_min=20, kept_max=40, sweeps=1, lanzero=1e-11, trun_err=1e-12,
+                 native_planner=False, resident=resident)
+        d.run()
+        runs.append([h[2] for h in d.history])
+    assert len(runs[0]) == len(runs[1])
+    assert np.allclose(runs[0], runs[1], atol=1e-9, rtol=0)

@@ -92,13 +92,28 @@ def test_truncating_step_matches_oracle(ctx):
     assert ospec == gspec
 
 
+@pytest.mark.parametrize("resident", [True, False])
 @pytest.mark.parametrize("Lx,Ny,Ne", [(3, 3, 8), (2, 4, 6)])
-def test_dmrg_untruncated_equals_ed(ctx, Lx, Ny, Ne):
+def test_dmrg_untruncated_equals_ed(ctx, Lx, Ny, Ne, resident):
     b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
     e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
-    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-10)
+    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-10, resident=resident)
     e = d.run()
     assert abs(e - e_ed) < 1e-8, (e, e_ed)
+
+
+def test_resident_dmrg_matches_host_operators(ctx):
+    """Operators kept on the device (merged and renormalised there) give the
+    energies of the host-operator driver, step by step, under truncation."""
+    runs = []
+    for resident in (True, False):
+        d = DMRG(ctx, 3, 4, Ne=10, kept_min=20, kept_max=40, sweeps=2, lanzero=1e-10, trun_err=1e-12,
+                 resident=resident)
+        runs.append((d.run(), [h[2] for h in d.history]))
+    (e_dev, steps_dev), (e_host, steps_host) = runs
+    assert len(steps_dev) == len(steps_host) > 0
+    np.testing.assert_allclose(steps_dev, steps_host, rtol=0, atol=1e-9)
+    assert abs(e_dev - e_host) < 1e-9, (e_dev, e_host)
 
 
 def test_dmrg_truncated_is_variational_and_converges_in_M(ctx):

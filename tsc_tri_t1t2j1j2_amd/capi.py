@@ -37,11 +37,12 @@ EXPORTS = [
     "tsc_last_error", "tsc_version", "tsc_ctx_create", "tsc_ctx_destroy", "tsc_ctx_sync", "tsc_ctx_mfma_layout",
     "tsc_vec_create", "tsc_vec_destroy", "tsc_vec_upload", "tsc_vec_download", "tsc_vec_dot_re",
     "tsc_vec_norm", "tsc_vec_dptr", "tsc_vec_fill_gaussian", "tsc_vec_axpby", "tsc_vec_copy",
-    "tsc_plan_create", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
+    "tsc_vec_copy_range", "tsc_vec_upload_range", "tsc_vec_download_range",
+    "tsc_plan_create", "tsc_plan_create_dev", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
     "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_graph", "tsc_plan_last_times",
     "tsc_lanczos", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate",
     "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_copy",
-    "tsc_tasks_plan", "tsc_tasks_destroy",
+    "tsc_tasks_plan", "tsc_tasks_plan_dev", "tsc_tasks_destroy",
 ]
 
 
@@ -86,6 +87,11 @@ def lib():
         "tsc_vec_fill_gaussian": (c_int, [c_void_p, c_uint64, c_uint64, c_double]),
         "tsc_vec_axpby": (c_int, [c_void_p, c_double, c_void_p, c_double]),
         "tsc_vec_copy": (c_int, [c_void_p, c_void_p]),
+        "tsc_vec_copy_range": (c_int, [c_void_p, c_int64, c_void_p, c_int64, c_int64]),
+        "tsc_vec_upload_range": (c_int, [c_void_p, c_int64, c_void_p, c_int64]),
+        "tsc_vec_download_range": (c_int, [c_void_p, c_int64, c_void_p, c_int64]),
+        "tsc_plan_create_dev": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_int32,
+                                        c_void_p, c_int32, c_void_p, c_int32, c_int64, c_int64, P(c_void_p)]),
         "tsc_plan_create": (c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_int32, c_void_p, c_int32,
                                     c_void_p, c_int32, c_void_p, c_int32, c_int64, c_int64, P(c_void_p)]),
         "tsc_plan_destroy": (c_int, [c_void_p]),
@@ -219,6 +225,22 @@ class Vec:
     def copy_from(self, x):
         check(lib().tsc_vec_copy(self.h, x.h))
 
+    def copy_range(self, off, src, src_off, n):
+        """self[off:off+n] = src[src_off:src_off+n] on the device (asynchronous)."""
+        check(lib().tsc_vec_copy_range(self.h, int(off), src.h, int(src_off), int(n)))
+
+    def upload_range(self, off, x):
+        """self[off:off+len(x)] = x; synchronous with respect to x."""
+        x = _c128(x).ravel(order="K")
+        check(lib().tsc_vec_upload_range(self.h, int(off), _ptr(x), x.size))
+        self.ctx.sync()
+
+    def download_range(self, off, n):
+        out = np.empty(int(n), dtype=np.complex128)
+        check(lib().tsc_vec_download_range(self.h, int(off), _ptr(out), int(n)))
+        self.ctx.sync()
+        return out
+
 
 class Plan:
     """A grouped two-stage contraction plan (H.psi, U^H O U, rho)."""
@@ -238,6 +260,25 @@ class Plan:
         self.in_elems = int(in_elems)
         self.out_elems = int(out_elems)
         self.ops_elems = int(ops_elems)
+        self.ops_vec = None
+
+    @classmethod
+    def create_dev(cls, ctx, ops_vec, mid_elems, c1, o1, c2, o2, in_elems, out_elems):
+        """Plan over an operator arena already on the device (borrowed: the
+        plan keeps a reference to ops_vec)."""
+        c1 = np.ascontiguousarray(c1, dtype=CONTRIB_DTYPE)
+        o1 = np.ascontiguousarray(o1, dtype=OUTBLOCK_DTYPE)
+        c2 = np.ascontiguousarray(c2, dtype=CONTRIB_DTYPE)
+        o2 = np.ascontiguousarray(o2, dtype=OUTBLOCK_DTYPE)
+        h = c_void_p()
+        check(lib().tsc_plan_create_dev(ctx.h, ops_vec.h, int(mid_elems), _ptr(c1), len(c1), _ptr(o1), len(o1),
+                                        _ptr(c2), len(c2), _ptr(o2), len(o2), int(in_elems), int(out_elems),
+                                        ctypes.byref(h)))
+        pl = cls.__new__(cls)
+        pl.ctx, pl.h = ctx, h
+        pl.in_elems, pl.out_elems, pl.ops_elems = int(in_elems), int(out_elems), ops_vec.n
+        pl.ops_vec = ops_vec
+        return pl
 
     def close(self):
         if self.h and self.ctx.h:

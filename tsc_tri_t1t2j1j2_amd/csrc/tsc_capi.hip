@@ -262,14 +262,16 @@ namespace tsc {
 // (padded with empty units, blk = -1, that exit at once).
 constexpr int kXcds = 8;  // MI355X: 256 CUs in 8 XCDs
 
-Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm, int tn, double desired) {
+Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm, int tn, double desired,
+                 const std::vector<char> *include) {
   auto ccost = [&](int32_t ci) {
     return (c[ci].a_op == TSC_OP_I || c[ci].b_op == TSC_OP_I) ? 1.0 : (double)std::max(c[ci].k, 1);
   };
   double total = 0.0;
   int64_t ntile = 0;
+  auto skip = [&](int32_t b) { return o[b].m == 0 || o[b].n == 0 || (include && !(*include)[b]); };
   for (int32_t b = 0; b < no; ++b) {
-    if (o[b].m == 0 || o[b].n == 0) continue;
+    if (skip(b)) continue;
     double w = 0.0;
     for (int32_t ci = o[b].cbeg; ci < o[b].cend; ++ci) w += ccost(ci);
     const int64_t nt = (int64_t)((o[b].m + tm - 1) / tm) * ((o[b].n + tn - 1) / tn);
@@ -288,7 +290,7 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm
   const double group_cap = total / (4.0 * kXcds);
   for (int32_t b = 0; b < no; ++b) {
     const tsc_outblock &ob = o[b];
-    if (ob.m == 0 || ob.n == 0) continue;
+    if (skip(b)) continue;
     double w = 0.0;
     std::pair<int, int64_t> key(1 << 30, 0);
     for (int32_t ci = ob.cbeg; ci < ob.cend; ++ci) {
@@ -473,6 +475,15 @@ extern "C" int tsc_plan_create(tsc_ctx *ctx, const double *ops, int64_t ops_elem
                                const tsc_contrib *c2, int32_t nc2, const tsc_outblock *o2, int32_t no2,
                                int64_t in_elems, int64_t out_elems, tsc_plan **out) {
   return plan_create_impl(ctx, ops, nullptr, ops_elems, mid_elems, c1, nc1, o1, no1, c2, nc2, o2, no2, in_elems,
+                          out_elems, out);
+}
+
+extern "C" int tsc_plan_create_dev(tsc_ctx *ctx, const tsc_vec *ops, int64_t mid_elems, const tsc_contrib *c1,
+                                   int32_t nc1, const tsc_outblock *o1, int32_t no1, const tsc_contrib *c2,
+                                   int32_t nc2, const tsc_outblock *o2, int32_t no2, int64_t in_elems,
+                                   int64_t out_elems, tsc_plan **out) {
+  TSC_REQUIRE(ctx && ops && ops->ctx == ctx, "tsc_plan_create_dev: operator arena of another context");
+  return plan_create_impl(ctx, nullptr, ops->d, ops->n, mid_elems, c1, nc1, o1, no1, c2, nc2, o2, no2, in_elems,
                           out_elems, out);
 }
 

@@ -118,7 +118,7 @@ struct PkStage {
   PkContrib *d_pc = nullptr;   // packed-path contribution records (permuted order)
   tsc_contrib *d_cp = nullptr; // the same contributions, original records (elementwise terms)
   PkOut *d_po = nullptr;       // stage-1 blocks: packed destinations
-  Tile *d_t = nullptr;
+  Tile *d_t = nullptr;         // 128x128 units
   Reduce *d_r = nullptr;
   int nt = 0, nr = 0;
   int64_t nslot = 0;
@@ -220,8 +220,9 @@ struct Units {
   std::vector<Reduce> reds;
   int64_t nslot = 0;
 };
+// include: optional per-block mask (blocks outside it get no units)
 Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm = kTM, int tn = kTN,
-                 double desired_units = 6144.0);
+                 double desired_units = 6144.0, const std::vector<char> *include = nullptr);
 double tridiag_lowest(const double *d, const double *e, int n);
 void tridiag_vector(const double *d, const double *e, int n, double lam, double *z);
 }  // namespace tsc

@@ -286,6 +286,7 @@ struct tsc_tasks {
   std::vector<tsc_outblock> o1, o2;
   int64_t mid_len = 0;
   int64_t psi_len = 0;
+  int64_t arena_len = 0;           // operator arena length the descriptors address
   std::vector<int32_t> super_xy;   // (x, y, dx, dy) per psi sector
   std::vector<int32_t> lsec, rsec; // (n, 2J, dim) of the enlarged bases
 };
@@ -343,6 +344,7 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
   cx.D = D;
   Basis L = get_basis(A), R = get_basis(Bv);
   tsc_tasks *T = new tsc_tasks();
+  T->arena_len = D->arena_len;
   // super basis
   std::vector<std::array<int, 4>> sb;
   std::map<std::pair<int, int>, int> sbi;
@@ -519,6 +521,14 @@ extern "C" int tsc_tasks_plan(tsc_ctx *ctx, const tsc_tasks *T, const double *ar
                               tsc_plan **plan) {
   TSC_REQUIRE(ctx && T && plan, "tsc_tasks_plan arguments");
   return tsc::plan_create_impl(ctx, arena, nullptr, arena_len, T->mid_len, T->c1.data(), (int32_t)T->c1.size(),
+                               T->o1.data(), (int32_t)T->o1.size(), T->c2.data(), (int32_t)T->c2.size(),
+                               T->o2.data(), (int32_t)T->o2.size(), T->psi_len, T->psi_len, plan);
+}
+
+extern "C" int tsc_tasks_plan_dev(tsc_ctx *ctx, const tsc_tasks *T, const tsc_vec *arena, tsc_plan **plan) {
+  TSC_REQUIRE(ctx && T && arena && plan && arena->ctx == ctx, "tsc_tasks_plan_dev arguments");
+  TSC_REQUIRE(arena->n >= T->arena_len, "tsc_tasks_plan_dev: arena shorter than the task descriptors need");
+  return tsc::plan_create_impl(ctx, nullptr, arena->d, arena->n, T->mid_len, T->c1.data(), (int32_t)T->c1.size(),
                                T->o1.data(), (int32_t)T->o1.size(), T->c2.data(), (int32_t)T->c2.size(),
                                T->o2.data(), (int32_t)T->o2.size(), T->psi_len, T->psi_len, plan);
 }

@@ -260,6 +260,30 @@ extern "C" int tsc_vec_axpby(tsc_vec *y, double a, const tsc_vec *x, double b) {
   return TSC_OK;
 }
 
+extern "C" int tsc_vec_copy_range(tsc_vec *y, int64_t y_off, const tsc_vec *x, int64_t x_off, int64_t n) {
+  TSC_REQUIRE(y && x && y->ctx == x->ctx && n >= 0 && y_off >= 0 && x_off >= 0 && y_off + n <= y->n &&
+                  x_off + n <= x->n,
+              "tsc_vec_copy_range arguments");
+  if (n == 0) return TSC_OK;
+  TSC_HIP_CHECK(hipMemcpyAsync(y->d + y_off, x->d + x_off, sizeof(double2) * n, hipMemcpyDeviceToDevice,
+                               y->ctx->stream));
+  return TSC_OK;
+}
+
+extern "C" int tsc_vec_upload_range(tsc_vec *v, int64_t off, const double *host, int64_t n) {
+  TSC_REQUIRE(v && (host || n == 0) && n >= 0 && off >= 0 && off + n <= v->n, "tsc_vec_upload_range arguments");
+  if (n == 0) return TSC_OK;
+  TSC_HIP_CHECK(hipMemcpyAsync(v->d + off, host, sizeof(double2) * n, hipMemcpyHostToDevice, v->ctx->stream));
+  return TSC_OK;
+}
+
+extern "C" int tsc_vec_download_range(const tsc_vec *v, int64_t off, double *host, int64_t n) {
+  TSC_REQUIRE(v && (host || n == 0) && n >= 0 && off >= 0 && off + n <= v->n, "tsc_vec_download_range arguments");
+  if (n == 0) return TSC_OK;
+  TSC_HIP_CHECK(hipMemcpyAsync(host, v->d + off, sizeof(double2) * n, hipMemcpyDeviceToHost, v->ctx->stream));
+  return TSC_OK;
+}
+
 extern "C" int tsc_vec_copy(tsc_vec *y, const tsc_vec *x) {
   TSC_REQUIRE(y && x && y->n == x->n && y->ctx == x->ctx, "tsc_vec_copy arguments");
   if (y->n == 0) return TSC_OK;

@@ -16,6 +16,7 @@ import time
 import numpy as np
 
 from . import capi
+from . import resident
 from . import hamiltonian as ham
 from . import lattice
 from . import native
@@ -43,7 +44,7 @@ def start_vector(super_basis, seed):
 class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
-                 prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True):
+                 prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -59,6 +60,9 @@ class DMRG:
         self.prec, self.refine = prec, refine
         self.seed = seed
         self.native_planner = native_planner
+        # block operators stay on the device between steps (resident.py)
+        self.resident = resident
+        self._dev_cache = {}
         self.bonds = lattice.bonds(Lx, Ny, t1, t2, J1, self.J2)
         self.nb = {}
         for (a, b, t, J) in self.bonds:
@@ -111,10 +115,23 @@ class DMRG:
                 order.append(x)
         return U, order, res["trun_err"], res["num_state"]
 
+    def _dev(self, model):
+        """The block with its operators on the device (uploaded once)."""
+        if getattr(model, "arena", None) is not None:
+            return model
+        d = self._dev_cache.get(id(model))
+        if d is None or d[0] is not model:
+            d = (model, resident.model_to_device(self.ctx, model))
+            self._dev_cache[id(model)] = d
+        return d[1]
+
     def step(self, sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys=False,
              identity_env=False, partners=()):
         """One DMRG step; grow: 'sys', 'env' or 'both'. Returns the energy and
         the grown block(s)."""
+        if self.resident:
+            return self._step_resident(sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err,
+                                       identity_sys, identity_env, partners)
         t0 = time.perf_counter()
         sb = ham.Superblock(sys, env, s1, s2, bonds, ntot, self.S2)
         if self.native_planner:
@@ -149,6 +166,59 @@ class DMRG:
             out["env_err"], out["env_kept"] = err, ns
         plan.close()
         psi.close()
+        self.nstep += 1
+        return out
+
+    def _step_resident(self, sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys,
+                       identity_env, partners):
+        """step() with every operator block on the device: the superblock
+        plan, the merged operators and the renormalisation all read one
+        StepArena, and the grown block is written to a new device arena."""
+        ctx = self.ctx
+        t0 = time.perf_counter()
+        sysd, envd = self._dev(sys), self._dev(env)
+        sb = ham.Superblock(sysd, envd, s1, s2, bonds, ntot, self.S2)
+        terms = sb.terms()
+        grows = {"sys": grow in ("sys", "both"), "env": grow in ("env", "both")}
+        spec = {}
+        for side, blk, site in (("sys", sysd, s1), ("env", envd, s2)):
+            if grows[side]:
+                to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(site, blk.sites, bonds)]
+                keep = self._keep_sites(blk.sites + [site], partners)
+                spec[side] = (blk, site, to_new, keep, ham.renorm_pairs(blk, to_new))
+        step = resident.StepArena(ctx, [sysd, envd])
+        step.add_merges(terms)
+        for v in spec.values():
+            step.add_merges(v[4])
+        step.finalize()
+        if self.native_planner:
+            tasks = native.NativeTasks(sb, step=step, terms=terms)
+            plan = tasks.plan(ctx)
+            tasks.close()
+        else:
+            plan = sb.plan_builder(resolver=step.offset, terms=terms).build_dev(ctx, step.vec, sb.super.dim,
+                                                                                sb.super.dim)
+        self.timers["plan"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        psi = ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
+        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=100, max_outer=4, prec=self.prec,
+                                   refine_iters=self.refine)
+        self.timers["lanczos"] += time.perf_counter() - t0
+        self.hpsi_flops += nmv * plan.flops()
+        out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
+        for side, idx, ident in (("sys", 0, identity_sys), ("env", 1, identity_env)):
+            if side not in spec:
+                continue
+            blk, site, to_new, keep, pairs = spec[side]
+            U, order, err, ns = self._truncate(sb, psi, idx, self.kept_min, kept_max, trun_err, ident)
+            t0 = time.perf_counter()
+            rp = ham.renormalize_plan(blk, site, to_new, keep, U, order, pairs=pairs, resolver=step.offset)
+            out[side] = resident.renormalize(ctx, step, rp)
+            self.timers["renorm"] += time.perf_counter() - t0
+            out[side + "_err"], out[side + "_kept"] = err, ns
+        plan.close()
+        psi.close()
+        step.close()
         self.nstep += 1
         return out
 

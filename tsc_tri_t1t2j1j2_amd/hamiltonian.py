@@ -29,6 +29,7 @@ from . import capi
 from . import su2
 from .blocks import SITE_SECTORS, SuperBasis, TotalBasis, TotalBlock, TotalModel, site_operators
 from .planner import IDENT, Operand, PlanBuilder
+from .resident import DevMat
 
 SQ2 = sqrt(2.0)
 SQ3 = sqrt(3.0)
@@ -155,7 +156,12 @@ def bond_products(t, J):
 
 
 def merged(ops_by_site, weights, name):
-    """sum_i w_i O_i[name] (block_add_block_two over the partner sites)."""
+    """sum_i w_i O_i[name] (block_add_block_two over the partner sites).
+    For blocks resident on the device the sum is left pending; the step's
+    StepArena forms it on the device."""
+    from .resident import merged_resident, resident
+    if any(resident(m) for i, _ in weights for m in ops_by_site[i][name].blocks.values()):
+        return merged_resident(ops_by_site, weights, name)
     acc = None
     for i, w in weights:
         o = ops_by_site[i][name]
@@ -274,11 +280,13 @@ class Superblock:
         return L_scalar_pairs, R_scalar_pairs, lr
 
     # ---- plan ----------------------------------------------------------
-    def plan_builder(self):
-        """PlanBuilder of model_wave for this superblock (psi in -> H psi)."""
+    def plan_builder(self, resolver=None, terms=None):
+        """PlanBuilder of model_wave for this superblock (psi in -> H psi).
+        resolver: (offset, ld) of a matrix in a device arena the plan will
+        borrow (resident blocks); terms: a precomputed self.terms()."""
         L, R, SB = self.sys_bs, self.env_bs, self.super
         A, B = self.sys, self.env
-        pb = PlanBuilder()
+        pb = PlanBuilder(resolver=resolver)
         # every target sub-block window, written exactly once
         win = {}
         for i, (x, y, dx, dy) in enumerate(SB.sub):
@@ -286,7 +294,7 @@ class Superblock:
             for (a, s, p, d) in L.sectors[x][3]:
                 for (b, t, q, e) in R.sectors[y][3]:
                     win[(i, p, q)] = pb.out_window((i, p, q), off + p + q * dx, dx, d, e)
-        Lps, Rps, lr = self.terms()
+        Lps, Rps, lr = self.terms() if terms is None else terms
         idL = identity_transitions(L)
         idR = identity_transitions(R)
         scalarL = scalar_transitions(L, A.sectors, ham=A.ham, pairs=Lps)
@@ -350,15 +358,10 @@ class RenormPlan:
     """Plan of one block renormalisation; see renormalize()."""
 
 
-def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order):
-    """Task arrays computing every operator of blk (x) new_site truncated by U.
-
-    bonds_to_new: [(i, t, J)] with i in blk; keep_sites: sites whose
-    operators the new block carries; U: {enlarged sector x: d_x x kept_x};
-    order: enlarged sectors of the new block, in their new order."""
-    basis = TotalBasis(blk.sectors)
+def renorm_pairs(blk, bonds_to_new):
+    """[T(block) x U(site)]^0 pairs of the enlarged block's Hamiltonian: the
+    block-site bonds, partner operators merged per operator type."""
     so = site_operators()
-    # H of the enlarged block: block H + merged block-site bonds
     pairs = []
     for name in range(4):
         ws, spec = [], None
@@ -371,6 +374,22 @@ def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order):
         if ws:
             pn, ptil, qn, qtil = spec
             pairs.append((1.0, merged(blk.ops, ws, pn), ptil, so[qn], qtil))
+    return pairs
+
+
+def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order, pairs=None, resolver=None):
+    """Task arrays computing every operator of blk (x) new_site truncated by U.
+
+    bonds_to_new: [(i, t, J)] with i in blk; keep_sites: sites whose
+    operators the new block carries; U: {enlarged sector x: d_x x kept_x};
+    order: enlarged sectors of the new block, in their new order; pairs: a
+    precomputed renorm_pairs(blk, bonds_to_new); resolver: as in
+    Superblock.plan_builder (resident blocks)."""
+    basis = TotalBasis(blk.sectors)
+    so = site_operators()
+    # H of the enlarged block: block H + merged block-site bonds
+    if pairs is None:
+        pairs = renorm_pairs(blk, bonds_to_new)
     trans = {"H": scalar_transitions(basis, blk.sectors, ham=blk.ham, pairs=pairs, reduced=True)}
     meta = {"H": (0, 0)}
     for i in keep_sites:
@@ -392,7 +411,7 @@ def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order):
         parts.append(u.ravel(order="F"))
         off += u.size
     uin = np.concatenate(parts) if parts else np.zeros(1, np.complex128)
-    pb = PlanBuilder()
+    pb = PlanBuilder(resolver=resolver)
     out_off = 0
     windows = {}
     for opkey, tl in trans.items():
@@ -414,6 +433,13 @@ def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order):
                 mo, mld = pb.op(mat[0], mat[1])
                 Z = pb.intermediate((mat[0], x, p), Operand(capi.BUF_OPS, mo, mld, mat[2]), ur, d2, kx, d)
                 pb.add(wid, ul, Z, d2, f)
+    # diagonal H blocks no term reaches stay zero (untouched output windows)
+    for x in order:
+        wk = ("H", x, x)
+        if wk not in windows:
+            kx = U[x].shape[1]
+            windows[wk] = (None, out_off)
+            out_off += kx * kx
     rp = RenormPlan()
     rp.builder, rp.uin, rp.out_len, rp.windows, rp.meta = pb, uin, max(out_off, 1), windows, meta
     rp.basis, rp.new_index, rp.order, rp.U = basis, new_index, order, U
@@ -422,8 +448,10 @@ def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order):
     return rp
 
 
-def renormalize_result(rp, res):
-    """TotalModel from the output buffer of a renormalisation plan."""
+def renormalize_result(rp, res, arena=None):
+    """TotalModel from the output buffer of a renormalisation plan: host
+    copies of `res`, or DevMat views of `arena` (the output vector left on
+    the device)."""
     basis, new_index, order, U = rp.basis, rp.new_index, rp.order, rp.U
     windows, meta, blk, new_site, keep_sites = rp.windows, rp.meta, rp.blk, rp.new_site, rp.keep_sites
     sectors = [(basis.sectors[x][0], basis.sectors[x][1], U[x].shape[1]) for x in order]
@@ -432,7 +460,10 @@ def renormalize_result(rp, res):
     for wk, (wid, o) in windows.items():
         opkey, x2, x = wk
         kx, kx2 = U[x].shape[1], U[x2].shape[1]
-        m = res[o:o + kx2 * kx].reshape((kx2, kx), order="F").copy()
+        if arena is None:
+            m = res[o:o + kx2 * kx].reshape((kx2, kx), order="F").copy()
+        else:
+            m = DevMat(arena, o, kx2, kx)
         if opkey == "H":
             if H is None:
                 H = TotalBlock(0, 0)

@@ -58,6 +58,8 @@ def _bind():
     L.tsc_tasks_copy.restype = ctypes.c_int
     L.tsc_tasks_plan.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_int64, P(ctypes.c_void_p)]
     L.tsc_tasks_plan.restype = ctypes.c_int
+    L.tsc_tasks_plan_dev.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_void_p)]
+    L.tsc_tasks_plan_dev.restype = ctypes.c_int
     L.tsc_tasks_destroy.argtypes = [ctypes.c_void_p]
     L.tsc_tasks_destroy.restype = ctypes.c_int
     L._tsc_native_bound = True
@@ -65,14 +67,19 @@ def _bind():
 
 
 class NativeTasks:
-    """Task arrays of one superblock built by the library's planner."""
+    """Task arrays of one superblock built by the library's planner.
 
-    def __init__(self, sb):
+    step: a finalized resident.StepArena holding every operator block (the
+    descriptors then carry its offsets and the plan borrows its device
+    vector; nothing is packed on the host); terms: a precomputed sb.terms()."""
+
+    def __init__(self, sb, step=None, terms=None):
         L = _bind()
-        Lps, Rps, lr = sb.terms()
+        Lps, Rps, lr = sb.terms() if terms is None else terms
         site_id = {id(o): _SITE_ID[k] for k, o in sb.site_ops.items()}
         arena, self._keep = [], []
-        self.arena_len = 0
+        self.step = step
+        self.arena_len = step.len if step is not None else 0
         op_index = {}
         descs = []
 
@@ -83,6 +90,9 @@ class NativeTasks:
             items = list(op.blocks.items())
             blks = (OpBlock * max(len(items), 1))()
             for n, ((b, k), m) in enumerate(items):
+                if step is not None:
+                    blks[n].bra, blks[n].ket, blks[n].off = b, k, step.offset(m)[0]
+                    continue
                 m = np.asarray(m, np.complex128)
                 blks[n].bra, blks[n].ket, blks[n].off = b, k, self.arena_len
                 arena.append(m.ravel(order="F"))
@@ -118,6 +128,8 @@ class NativeTasks:
         capi.check(L.tsc_superblock_tasks(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
         self.arena = np.concatenate(arena) if arena else np.zeros(1, np.complex128)
+        if step is not None:
+            self.arena = None
         cnt = (c_int64 * 9)()
         capi.check(L.tsc_tasks_info(h, cnt))
         self.counts = list(cnt)
@@ -140,12 +152,18 @@ class NativeTasks:
     def plan(self, ctx):
         L = _bind()
         h = ctypes.c_void_p()
-        capi.check(L.tsc_tasks_plan(ctx.h, self.h, self.arena.ctypes.data_as(ctypes.c_void_p), self.arena.size,
-                                    ctypes.byref(h)))
         pl = capi.Plan.__new__(capi.Plan)
+        pl.ops_vec = None
+        if self.step is not None:
+            capi.check(L.tsc_tasks_plan_dev(ctx.h, self.h, self.step.vec.h, ctypes.byref(h)))
+            pl.ops_vec = self.step.vec
+            pl.ops_elems = int(self.step.vec.n)
+        else:
+            capi.check(L.tsc_tasks_plan(ctx.h, self.h, self.arena.ctypes.data_as(ctypes.c_void_p), self.arena.size,
+                                        ctypes.byref(h)))
+            pl.ops_elems = int(self.arena.size)
         pl.ctx, pl.h = ctx, h
         pl.in_elems = pl.out_elems = int(self.psi_len)
-        pl.ops_elems = int(self.arena.size)
         return pl
 
     def close(self):

@@ -27,7 +27,10 @@ IDENT = None
 
 
 class PlanBuilder:
-    def __init__(self):
+    def __init__(self, resolver=None):
+        # resolver(mat) -> (offset, ld) in a device arena the plan borrows;
+        # None: matrices are copied into the plan's own host arena
+        self.resolver = resolver
         self.arena = []
         self.arena_len = 0
         self.arena_index = {}
@@ -43,6 +46,10 @@ class PlanBuilder:
         """Arena offset and leading dimension of a stored matrix."""
         if key in self.arena_index:
             return self.arena_index[key]
+        if self.resolver is not None:
+            res = self.resolver(mat)
+            self.arena_index[key] = res
+            return res
         m = np.asarray(mat, dtype=np.complex128)
         off = self.arena_len
         self.arena.append(m.ravel(order="F"))
@@ -125,3 +132,8 @@ class PlanBuilder:
         c1, o1, c2, o2 = self.arrays()
         ops = self.arena_array()
         return capi.Plan(ctx, ops, ops.size, self.mid_len, c1, o1, c2, o2, int(in_len), int(out_len))
+
+    def build_dev(self, ctx, ops_vec, in_len, out_len):
+        """Plan over a device arena (the resolver's offsets)."""
+        c1, o1, c2, o2 = self.arrays()
+        return capi.Plan.create_dev(ctx, ops_vec, self.mid_len, c1, o1, c2, o2, int(in_len), int(out_len))
