@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--lanczos-iters", type=int, default=40)
     ap.add_argument("--refine", type=int, default=8, help="FP64 Lanczos steps after the mixed ones")
     ap.add_argument("--no-dmrg", action="store_true", help="skip the FP64-vs-mixed DMRG energy check")
+    ap.add_argument("--no-full-step", action="store_true", help="skip the resident full-step timing")
     ap.add_argument("--dmrg-sweeps", type=int, default=6)
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "f32", "f64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -69,6 +70,8 @@ class PhysicsWorkload:
         from tsc_tri_t1t2j1j2_amd import workloads
         t0 = time.perf_counter()
         sysb, envb, s1, s2, bonds, ntot = workloads.centre_superblock(Lx, Ny, M, seed=seed)
+        self.blocks = (sysb, envb, s1, s2, bonds, ntot)
+        self.Lx, self.Ny, self.M = Lx, Ny, M
         self.sb = ham.Superblock(sysb, envb, s1, s2, bonds, ntot, 0)
         self.pb = self.sb.plan_builder()
         self.c1, self.o1, self.c2, self.o2 = self.pb.arrays()
@@ -187,19 +190,56 @@ def lanczos_and_step(ctx, plan, wl, vin, prec, args):
     sd = [dx for (_, _, dx, _) in sbk.super.sub]
     ed = [dy for (_, _, _, dy) in sbk.super.sub]
     twos = [sbk.sys_bs.sectors[x][1] for (x, _, _, _) in sbk.super.sub]
+    # the first call also creates the solver handles and streams: time the second
+    capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, 6000, 1e-15)
     ctx.sync()
     t0 = time.perf_counter()
     tr = capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, 6000, 1e-15)
     ttr = time.perf_counter() - t0
-    nsteps = 2 * (wl.N - 3)
     s64, smx = t64 + ttr, tm + ttr
     return {"lanczos_budget": f"FP64 {it} H.psi; mixed {it} + {refine} FP64 refinement H.psi",
             "lanczos_f64_s": round(t64, 3), "lanczos_mixed_s": round(tm, 3), "rho_truncate_s": round(ttr, 3),
-            "kept": int(tr["num_state"]), "step_s_f64": round(s64, 3), "step_s_mixed": round(smx, 3),
+            "kept": int(tr["num_state"]), "lanczos_rho_s_f64": round(s64, 3), "lanczos_rho_s_mixed": round(smx, 3)}
+
+
+def full_step(ctx, wl, prec, args):
+    """Wall time of one whole sweep step on the centre superblock with the
+    block operators resident on the device, as dmrg.DMRG runs it: step arena
+    + merged operators, superblock plan, Lanczos at the fixed budget of
+    lanczos_and_step(), rho / eigensolve / truncation of the system side, and
+    the renormalised (enlarged + truncated) system block written to a new
+    device arena. The first upload of the two blocks is timed apart: in a
+    sweep they are already resident."""
+    from tsc_tri_t1t2j1j2_amd import capi
+    from tsc_tri_t1t2j1j2_amd.dmrg import DMRG
+    sysb, envb, s1, s2, bonds, ntot = wl.blocks
+    out = {}
+    for name, p, rf in (("f64", capi.PREC_F64, 0), ("mixed", prec, args.refine)):
+        d = DMRG(ctx, wl.Lx, wl.Ny, kept_min=110, kept_max=wl.M, prec=p, refine=rf, resident=True,
+                 lanczos_budget=(args.lanczos_iters, 0))
+        t0 = time.perf_counter()
+        sysd, envd = d._dev(sysb), d._dev(envb)
+        ctx.sync()
+        tup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = d.step(sysd, envd, s1, s2, bonds, ntot, 1e-13, "sys", wl.M, 1e-15)
+        ctx.sync()
+        tstep = time.perf_counter() - t0
+        tm = {k: round(v, 3) for k, v in d.timers.items()}
+        print(f"# full step {name}: {tstep:.2f}s {tm} kept {r['sys_kept']} E={r['E']:.10f} "
+              f"(upload of both blocks {tup:.2f}s)", file=sys.stderr, flush=True)
+        out[name] = (tstep, tm, r["sys_kept"], tup)
+        del r, d, sysd, envd
+    nsteps = 2 * (wl.N - 3)
+    s64, smx = out["f64"][0], out["mixed"][0]
+    return {"full_step_s_f64": round(s64, 3), "full_step_s_mixed": round(smx, 3),
+            "full_step_parts_f64": out["f64"][1], "full_step_parts_mixed": out["mixed"][1],
+            "full_step_kept": int(out["mixed"][2]), "block_upload_s": round(out["mixed"][3], 3),
             "sweep_s_f64_estimate": round(s64 * nsteps, 1), "sweep_s_mixed_estimate": round(smx * nsteps, 1),
             "sweep_speedup_mixed_vs_f64": round(s64 / smx, 3),
-            "sweep_estimate_note": f"{nsteps} centre-sized steps x (Lanczos + rho/truncation); plan build, "
-                                   "renormalisation and the wave-function guess excluded"}
+            "sweep_estimate_note": f"{nsteps} centre-sized steps x one measured full step (step arena and merges, "
+                                   "plan, Lanczos at the fixed budget, rho/truncation, renormalisation); the "
+                                   "wave-function guess is not implemented (random start vector)"}
 
 
 def dmrg_precision(ctx, sweeps, refine):
@@ -315,6 +355,9 @@ def main():
     extra = {}
     if isinstance(sb, PhysicsWorkload) and not args.no_lanczos:
         extra = lanczos_and_step(ctx, plan, sb, vin, prec, args)
+        if not args.no_full_step:
+            plan.close()
+            extra.update(full_step(ctx, sb, prec, args))
     if isinstance(sb, PhysicsWorkload) and not (args.no_lanczos or args.no_dmrg) and prec != capi.PREC_F64:
         extra.update(dmrg_precision(ctx, args.dmrg_sweeps, args.refine))
     cpu = None

@@ -44,7 +44,8 @@ def start_vector(super_basis, seed):
 class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
-                 prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True):
+                 prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True,
+                 lanczos_budget=(100, 4)):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -58,6 +59,8 @@ class DMRG:
         self.lanzero = lanzero
         self.trun_err = trun_err
         self.prec, self.refine = prec, refine
+        # (inner iterations, restarts) of each solve: lanczos_memo_new's loop_in=100, loop_out=4 (eigen.f90:14)
+        self.lanczos_budget = tuple(int(v) for v in lanczos_budget)
         self.seed = seed
         self.native_planner = native_planner
         # block operators stay on the device between steps (resident.py)
@@ -143,7 +146,8 @@ class DMRG:
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         psi = self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
-        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=100, max_outer=4, prec=self.prec,
+        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
+                                   max_outer=self.lanczos_budget[1], prec=self.prec,
                                    refine_iters=self.refine)
         self.timers["lanczos"] += time.perf_counter() - t0
         self.hpsi_flops += nmv * plan.flops()
@@ -201,7 +205,8 @@ class DMRG:
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         psi = ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
-        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=100, max_outer=4, prec=self.prec,
+        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
+                                   max_outer=self.lanczos_budget[1], prec=self.prec,
                                    refine_iters=self.refine)
         self.timers["lanczos"] += time.perf_counter() - t0
         self.hpsi_flops += nmv * plan.flops()
