@@ -329,7 +329,8 @@ def main():
 
     # FP64 reference application and timing
     w64, k1_64, k2_64 = time_gpu(ctx, plan, vin, vref, capi.PREC_F64, args.steps, args.warmup)
-    print(f"# f64: {w64 * 1e3:.2f} ms/step, {F / w64 / 1e12:.2f} TFLOP/s", file=sys.stderr, flush=True)
+    print(f"# f64: {w64 * 1e3:.2f} ms/step, {F / w64 / 1e12:.2f} TFLOP/s (stage 1 {k1_64:.2f} ms, stage 2 "
+          f"{k2_64:.2f} ms)", file=sys.stderr, flush=True)
     ref = vref.download()
     if prec != capi.PREC_F64:
         wall, k1, k2 = time_gpu(ctx, plan, vin, vout, prec, args.steps, args.warmup)
@@ -337,8 +338,8 @@ def main():
         rel = float(np.linalg.norm(out - ref) / np.linalg.norm(ref))
     else:
         wall, k1, k2, rel = w64, k1_64, k2_64, 0.0
-    print(f"# {args.prec}: {wall * 1e3:.2f} ms/step, {F / wall / 1e12:.2f} TFLOP/s, rel err {rel:.2e}",
-          file=sys.stderr, flush=True)
+    print(f"# {args.prec}: {wall * 1e3:.2f} ms/step, {F / wall / 1e12:.2f} TFLOP/s, rel err {rel:.2e} (stage 1 "
+          f"{k1:.2f} ms, stage 2 {k2:.2f} ms)", file=sys.stderr, flush=True)
 
     # dominant kernel roofline (MFMA-bound)
     if k1 >= k2:

@@ -386,9 +386,9 @@ extern "C" int tsc_plan_destroy(tsc_plan *p) {
     hipFree(p->d_t[s]);
     hipFree(p->d_r[s]);
   }
-  hipFree(p->d_part64);
   hipFree(p->d_part32);
   tsc::pk_free(p->pk);
+  tsc::gd_free(p->gd);
   for (auto v : p->kry) hipFree(v);
   for (int i = 0; i < 3; ++i) {
     if (p->ev[i]) hipEventDestroy(p->ev[i]);
@@ -535,15 +535,14 @@ int ensure_precision(tsc_plan *p, int prec) {
     }
     return pk_build(p);
   }
-  const int64_t slots = std::max(p->nslot[0], p->nslot[1]);
-  if (slots > 0) {
-    void **part = (prec == TSC_PREC_F64) ? &p->d_part64 : &p->d_part32;
-    const size_t esz = (prec == TSC_PREC_F64) ? sizeof(double2) : sizeof(float2);
-    if (!*part) TSC_HIP_CHECK(hipMalloc(part, esz * (size_t)kTM * kTN * (size_t)slots));
-  }
   if (prec == TSC_PREC_F64) {
     if (!p->d_mid64 && p->mid_elems > 0) TSC_HIP_CHECK(hipMalloc((void **)&p->d_mid64, sizeof(double2) * p->mid_elems));
-  } else {
+    return gd_build(p);
+  }
+  const int64_t slots = std::max(p->nslot[0], p->nslot[1]);
+  if (slots > 0 && !p->d_part32)
+    TSC_HIP_CHECK(hipMalloc(&p->d_part32, sizeof(float2) * (size_t)kTM * kTN * (size_t)slots));
+  {
     if (!p->d_ops32 && p->ops_elems > 0) {
       TSC_HIP_CHECK(hipMalloc((void **)&p->d_ops32, sizeof(float2) * p->ops_elems));
       hipLaunchKernelGGL(f64_to_f32_kernel, dim3(4096), dim3(256), 0, s, p->d_ops64, p->d_ops32, p->ops_elems);
@@ -557,13 +556,14 @@ int ensure_precision(tsc_plan *p, int prec) {
 int enqueue_hpsi(tsc_plan *p, const double2 *in, double2 *out, int prec, bool events) {
   hipStream_t s = p->ctx->stream;
   if (prec == TSC_PREC_BF16X3) return pk_enqueue(p, in, out, events);
+  if (prec == TSC_PREC_F64) return gd_enqueue(p, in, out, events);
   Bufs b;
-  b.p[TSC_BUF_OPS] = (prec == TSC_PREC_F64) ? (const void *)p->d_ops64 : (const void *)p->d_ops32;
+  b.p[TSC_BUF_OPS] = p->d_ops32;
   b.p[TSC_BUF_IN] = in;
-  b.p[TSC_BUF_MID] = (prec == TSC_PREC_F64) ? (const void *)p->d_mid64 : (const void *)p->d_mid32;
+  b.p[TSC_BUF_MID] = p->d_mid32;
   b.p[TSC_BUF_OUT] = out;
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[0], s));
-  void *part = (prec == TSC_PREC_F64) ? p->d_part64 : p->d_part32;
+  void *part = p->d_part32;
   TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[0], p->d_c[0], p->d_t[0], p->nt[0], p->d_r[0], p->nr[0], part, s,
                                p->ctx->mfma_rowmode));
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[1], s));

@@ -17,10 +17,10 @@
 //
 // Waves are arranged 2x2 over the tile; each wave computes a 32x32 sub-tile
 // as 2x2 MFMA 16x16 blocks, real and imaginary parts in separate
-// accumulators (4 real MFMAs per complex k-group).
-//   FP64  : v_mfma_f64_16x16x4f64, TK = 16
-//   FP32  : v_mfma_f32_16x16x4f32, TK = 16 (complex FP32 products)
-// The BF16x3 path has kernels of its own (packed operands, tsc_pk.hip).
+// accumulators (4 real MFMAs per complex k-group), here with
+// v_mfma_f32_16x16x4f32, TK = 16 (complex FP32 products: the FP32 path).
+// The FP64 path (tsc_gd.hip) and the BF16x3 path (tsc_pk.hip) have kernels
+// of their own; this file also holds the MFMA layout probes.
 #include "tsc_internal.h"
 
 namespace tsc {
@@ -505,24 +505,15 @@ hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_bloc
                           hipStream_t stream, const int *modes) {
   if (ntiles <= 0) return hipSuccess;
   dim3 grid(ntiles), block(256);
-  if (prec == TSC_PREC_F64) {
-    hipLaunchKernelGGL((grouped_kernel<double, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
-                       modes[0], part);
-  } else if (prec == TSC_PREC_F32) {
-    hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
-                       modes[1], part);
-  } else {
-    return hipErrorNotSupported;  // BF16x3 runs on the packed-operand kernels
-  }
+  // FP64 runs on the direct-to-LDS kernels (tsc_gd.hip), BF16x3 on the
+  // packed-operand ones (tsc_pk.hip)
+  if (prec != TSC_PREC_F32) return hipErrorNotSupported;
+  hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
+                     modes[1], part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nreds <= 0) return e;
-  if (prec == TSC_PREC_F64) {
-    hipLaunchKernelGGL(reduce_kernel<double>, dim3(nreds), dim3(256), 0, stream, bufs, d_blocks, d_reds,
-                       (const void *)part);
-  } else {
-    hipLaunchKernelGGL(reduce_kernel<float>, dim3(nreds), dim3(256), 0, stream, bufs, d_blocks, d_reds,
-                       (const void *)part);
-  }
+  hipLaunchKernelGGL(reduce_kernel<float>, dim3(nreds), dim3(256), 0, stream, bufs, d_blocks, d_reds,
+                     (const void *)part);
   return hipGetLastError();
 }
 

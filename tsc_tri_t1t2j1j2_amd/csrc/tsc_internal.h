@@ -137,6 +137,21 @@ struct PkState {
 
 void pk_free(PkState *s);
 
+// FP64 path (tsc_gd.hip): 128x128 units of 8 waves over the plan's own
+// complex128 buffers.
+constexpr int kGdTile = 128;
+struct GdStage {
+  tsc_contrib *d_c = nullptr;  // contributions, each block's GEMM terms first
+  Tile *d_t = nullptr;
+  Reduce *d_r = nullptr;
+  int nt = 0, nr = 0;
+};
+struct GdState {
+  GdStage st[2];
+  double2 *d_part = nullptr;  // partial tiles of split units
+};
+void gd_free(GdState *s);
+
 // Tile shape of the grouped contraction kernels.
 constexpr int kTM = 64;
 constexpr int kTN = 64;
@@ -186,8 +201,7 @@ struct tsc_plan {
   tsc::Reduce *d_r[2] = {nullptr, nullptr};
   int nr[2] = {0, 0};
   int64_t nslot[2] = {0, 0};           // partial-tile slots per stage
-  void *d_part64 = nullptr;            // partial tiles, FP64 path
-  void *d_part32 = nullptr;            // partial tiles, FP32 / BF16x3 paths
+  void *d_part32 = nullptr;            // partial tiles, FP32 path
   double flops = 0.0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool timed = false;
@@ -199,6 +213,7 @@ struct tsc_plan {
   std::vector<tsc_contrib> h_c[2];
   std::vector<tsc_outblock> h_o[2];
   tsc::PkState *pk = nullptr;
+  tsc::GdState *gd = nullptr;
   // Lanczos Krylov vectors (kry_n elements each), kept between calls
   std::vector<double2 *> kry;
   int64_t kry_n = 0;
@@ -214,6 +229,8 @@ int hpsi_dev(tsc_plan *p, const double2 *in, double2 *out, int prec);
 int pk_build(tsc_plan *p);         // build on first use; packs the operator arena
 int pk_repack_ops(tsc_plan *p);    // after the arena changed
 int pk_enqueue(tsc_plan *p, const double2 *in, double2 *out, bool events);
+int gd_build(tsc_plan *p);  // FP64 units and contribution order, on first use
+int gd_enqueue(tsc_plan *p, const double2 *in, double2 *out, bool events);
 // work units of one stage (shared by all paths)
 struct Units {
   std::vector<Tile> tiles;
