@@ -243,24 +243,35 @@ def full_step(ctx, wl, prec, args):
 
 
 def dmrg_precision(ctx, sweeps, refine):
-    """|dE|/site between an FP64 and a mixed (BF16x3 + `refine` FP64 Lanczos
-    steps per solve) run of the full DMRG on YC4 4x6 (24 sites, 22 electrons,
-    kept_max 256)."""
+    """|dE|/site of the mixed precision (BF16x3 + `refine` FP64 Lanczos steps
+    per solve) on YC4 4x6 (24 sites, 22 electrons, kept_max 256), two ways:
+    (1) two whole runs, FP64 and mixed: their difference also contains the
+    effect of different truncation choices along the two paths; (2) along the
+    mixed run, every superblock is also solved in FP64 from the same start
+    vector: the largest per-step difference is the precision of the mixed
+    solve itself."""
     from tsc_tri_t1t2j1j2_amd import capi
     from tsc_tri_t1t2j1j2_amd.dmrg import DMRG
     out = {}
+    shadow = []
     for name, prec, rf in (("f64", capi.PREC_F64, 0), ("bf16x3", capi.PREC_BF16X3, refine)):
         t0 = time.perf_counter()
         d = DMRG(ctx, 6, 4, kept_min=110, kept_max=256, sweeps=sweeps, lanzero=1e-10, prec=prec, refine=rf,
-                 seed=1234)
+                 seed=1234, shadow_f64=(prec != capi.PREC_F64))
         E = d.run()
         out[name] = (E, time.perf_counter() - t0, d.N)
+        shadow = shadow or d.shadow
         print(f"# dmrg YC4 6x4 M=256 {name}: E={E:.12f} in {out[name][1]:.1f}s", file=sys.stderr, flush=True)
     N = out["f64"][2]
+    per_step = [abs(em - e64) / N for (_, em, e64) in shadow]
     return {"dmrg_check": f"YC4 4x6 (24 sites, 22 electrons) kept_max 256, {sweeps} sweeps, lanzero 1e-10",
             "dmrg_E_per_site_f64": out["f64"][0] / N, "dmrg_E_per_site_mixed": out["bf16x3"][0] / N,
             "dmrg_dE_per_site_mixed_vs_f64": abs(out["f64"][0] - out["bf16x3"][0]) / N, "dmrg_refine": refine,
-            "dmrg_wall_s_f64": round(out["f64"][1], 2), "dmrg_wall_s_mixed": round(out["bf16x3"][1], 2)}
+            "dmrg_same_path_steps": len(per_step),
+            "dmrg_same_path_max_dE_per_site": max(per_step) if per_step else None,
+            "dmrg_same_path_last_dE_per_site": per_step[-1] if per_step else None,
+            "dmrg_wall_s_f64": round(out["f64"][1], 2),
+            "dmrg_wall_s_mixed_with_fp64_shadow": round(out["bf16x3"][1], 2)}
 
 
 def cpu_baseline(sb, psi_host, seconds):

@@ -45,7 +45,7 @@ class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
                  prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True,
-                 lanczos_budget=(100, 4)):
+                 lanczos_budget=(100, 4), shadow_f64=False):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -61,6 +61,11 @@ class DMRG:
         self.prec, self.refine = prec, refine
         # (inner iterations, restarts) of each solve: lanczos_memo_new's loop_in=100, loop_out=4 (eigen.f90:14)
         self.lanczos_budget = tuple(int(v) for v in lanczos_budget)
+        # shadow_f64: with a mixed precision, also solve every superblock in
+        # FP64 from the same start vector (the sweep follows the mixed
+        # solution); self.shadow holds (step, E_mixed, E_f64)
+        self.shadow_f64 = shadow_f64
+        self.shadow = []
         self.seed = seed
         self.native_planner = native_planner
         # block operators stay on the device between steps (resident.py)
@@ -118,6 +123,15 @@ class DMRG:
                 order.append(x)
         return U, order, res["trun_err"], res["num_state"]
 
+    def _shadow(self, sb, plan, tol, E):
+        if not self.shadow_f64 or self.prec == capi.PREC_F64:
+            return
+        psi = self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
+        E64, _ = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0], max_outer=self.lanczos_budget[1],
+                              prec=capi.PREC_F64)
+        psi.close()
+        self.shadow.append((self.nstep, E, E64))
+
     def _dev(self, model):
         """The block with its operators on the device (uploaded once)."""
         if getattr(model, "arena", None) is not None:
@@ -150,6 +164,7 @@ class DMRG:
                                    max_outer=self.lanczos_budget[1], prec=self.prec,
                                    refine_iters=self.refine)
         self.timers["lanczos"] += time.perf_counter() - t0
+        self._shadow(sb, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
         if grow in ("sys", "both"):
@@ -209,6 +224,7 @@ class DMRG:
                                    max_outer=self.lanczos_budget[1], prec=self.prec,
                                    refine_iters=self.refine)
         self.timers["lanczos"] += time.perf_counter() - t0
+        self._shadow(sb, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
         for side, idx, ident in (("sys", 0, identity_sys), ("env", 1, identity_env)):
