@@ -30,6 +30,15 @@ class FakeVec:
     def download_range(self, off, n):
         return self.x[off:off + n].copy()
 
+    def norm(self):
+        return float(np.linalg.norm(self.x))
+
+    def axpby(self, a, x, b):
+        self.x[:] = a * x.x + b * self.x
+
+    def copy_from(self, x):
+        self.x[:] = x.x
+
     def close(self):
         pass
 

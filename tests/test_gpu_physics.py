@@ -129,3 +129,36 @@ def test_dmrg_truncated_is_variational_and_converges_in_M(ctx):
     assert es[0] >= es[1] - 1e-8 >= es[2] - 2e-8, es
     assert (es[1] - e_ed) / (Lx * Ny) < 1e-3
     assert (es[2] - e_ed) / (Lx * Ny) < 1e-7, (es[2], e_ed)
+
+
+def test_wave_guess_on_device(ctx, monkeypatch):
+    """The transformed previous ground state (wave_trans_LR/RL) is the new
+    superblock's ground state when nothing is truncated, and the guessed
+    solves need only a few H.psi."""
+    starts, finals, mvs = [], [], []
+    orig_initial = DMRG._initial
+
+    def initial(self, sb, guess):
+        v = orig_initial(self, sb, guess)
+        starts.append((guess is not None, v.download()))
+        return v
+
+    monkeypatch.setattr(DMRG, "_initial", initial)
+    orig_step = DMRG.step
+
+    def step(self, *a, **k):
+        r = orig_step(self, *a, **k)
+        finals.append(r["psi"].download())
+        mvs.append(r["matvecs"])
+        return r
+
+    monkeypatch.setattr(DMRG, "step", step)
+    d = DMRG(ctx, 3, 3, Ne=8, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-11)
+    E = d.run()
+    ovs = [abs(np.vdot(f, v)) / (np.linalg.norm(f) * np.linalg.norm(v)) for (g, v), f in zip(starts, finals) if g]
+    assert d.guessed == len(ovs) > 8
+    assert min(ovs) > 1 - 1e-9, ovs
+    guessed_mv = [m for (g, _), m in zip(starts, mvs) if g]
+    assert max(guessed_mv) <= 6, guessed_mv
+    e_ed = tj_ed.ground_energy(9, 8, tj_model.bonds(3, 3, 1.0, 0.2, 0.4, 0.016))
+    assert abs(E - e_ed) < 1e-8

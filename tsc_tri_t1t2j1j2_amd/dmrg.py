@@ -15,11 +15,14 @@ import time
 
 import numpy as np
 
+from types import SimpleNamespace
+
 from . import capi
 from . import resident
 from . import hamiltonian as ham
 from . import lattice
 from . import native
+from . import wavetrans
 from .blocks import SITE_SECTORS, TotalBlock, TotalModel, site_operators
 
 
@@ -45,7 +48,7 @@ class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
                  prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True,
-                 lanczos_budget=(100, 4), shadow_f64=False):
+                 lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -66,6 +69,10 @@ class DMRG:
         # solution); self.shadow holds (step, E_mixed, E_f64)
         self.shadow_f64 = shadow_f64
         self.shadow = []
+        # wave_guess: inside a half-sweep, start each Lanczos from the previous
+        # ground state carried into the new superblock (wave_trans_LR/RL)
+        self.wave_guess = wave_guess
+        self.sys_trun, self.env_trun = {}, {}   # block length -> wavetrans.Trunc that built it
         self.seed = seed
         self.native_planner = native_planner
         # block operators stay on the device between steps (resident.py)
@@ -82,7 +89,8 @@ class DMRG:
         self.log = log
         self.nstep = 0
         self.hpsi_flops = 0.0
-        self.timers = {"plan": 0.0, "lanczos": 0.0, "truncate": 0.0, "renorm": 0.0}
+        self.timers = {"plan": 0.0, "guess": 0.0, "lanczos": 0.0, "truncate": 0.0, "renorm": 0.0}
+        self.guessed = 0
 
     # ---- helpers ---------------------------------------------------------
     def _say(self, msg):
@@ -123,13 +131,32 @@ class DMRG:
                 order.append(x)
         return U, order, res["trun_err"], res["num_state"]
 
-    def _shadow(self, sb, plan, tol, E):
+    def _initial(self, sb, guess):
+        """Lanczos start: the transformed previous ground state when the
+        sweep provides one (wave_trans_LR/RL), else the seeded random vector
+        (wave_initialize)."""
+        if guess is not None:
+            t0 = time.perf_counter()
+            v = guess(sb)
+            self.timers["guess"] += time.perf_counter() - t0
+            if v is not None:
+                self.guessed += 1
+                return v
+        return self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
+
+    def _shadow_start(self, psi):
         if not self.shadow_f64 or self.prec == capi.PREC_F64:
+            return None
+        v = self.ctx.vec(psi.n)
+        v.copy_from(psi)
+        return v
+
+    def _shadow(self, psi0, plan, tol, E):
+        if psi0 is None:
             return
-        psi = self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
-        E64, _ = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0], max_outer=self.lanczos_budget[1],
+        E64, _ = plan.lanczos(psi0, tol=tol, max_inner=self.lanczos_budget[0], max_outer=self.lanczos_budget[1],
                               prec=capi.PREC_F64)
-        psi.close()
+        psi0.close()
         self.shadow.append((self.nstep, E, E64))
 
     def _dev(self, model):
@@ -143,12 +170,14 @@ class DMRG:
         return d[1]
 
     def step(self, sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys=False,
-             identity_env=False, partners=()):
-        """One DMRG step; grow: 'sys', 'env' or 'both'. Returns the energy and
-        the grown block(s)."""
+             identity_env=False, partners=(), guess=None):
+        """One DMRG step; grow: 'sys', 'env' or 'both'. Returns the energy,
+        the grown block(s) with the truncations that built them, and the
+        ground state (device vector) with its superblock bases. guess(sb):
+        Lanczos start vector for superblock sb, or None."""
         if self.resident:
             return self._step_resident(sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err,
-                                       identity_sys, identity_env, partners)
+                                       identity_sys, identity_env, partners, guess)
         t0 = time.perf_counter()
         sb = ham.Superblock(sys, env, s1, s2, bonds, ntot, self.S2)
         if self.native_planner:
@@ -159,12 +188,13 @@ class DMRG:
             plan = sb.plan_builder().build(self.ctx, sb.super.dim, sb.super.dim)
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        psi = self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
+        psi = self._initial(sb, guess)
+        psi0 = self._shadow_start(psi)
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
                                    max_outer=self.lanczos_budget[1], prec=self.prec,
                                    refine_iters=self.refine)
         self.timers["lanczos"] += time.perf_counter() - t0
-        self._shadow(sb, plan, tol, E)
+        self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
         if grow in ("sys", "both"):
@@ -173,6 +203,7 @@ class DMRG:
             to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(s1, sys.sites, bonds)]
             keep = self._keep_sites(sys.sites + [s1], partners)
             out["sys"] = ham.renormalize(self.ctx, sys, s1, to_new, keep, U, order)
+            out["sys_trun"] = wavetrans.Trunc(sb.sys_bs, U, order)
             self.timers["renorm"] += time.perf_counter() - t0
             out["sys_err"], out["sys_kept"] = err, ns
         if grow in ("env", "both"):
@@ -181,15 +212,17 @@ class DMRG:
             to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(s2, env.sites, bonds)]
             keep = self._keep_sites(env.sites + [s2], partners)
             out["env"] = ham.renormalize(self.ctx, env, s2, to_new, keep, U, order)
+            out["env_trun"] = wavetrans.Trunc(sb.env_bs, U, order)
             self.timers["renorm"] += time.perf_counter() - t0
             out["env_err"], out["env_kept"] = err, ns
         plan.close()
-        psi.close()
+        out["psi"] = psi
+        out["bases"] = SimpleNamespace(super=sb.super, sys_bs=sb.sys_bs, env_bs=sb.env_bs)
         self.nstep += 1
         return out
 
     def _step_resident(self, sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys,
-                       identity_env, partners):
+                       identity_env, partners, guess=None):
         """step() with every operator block on the device: the superblock
         plan, the merged operators and the renormalisation all read one
         StepArena, and the grown block is written to a new device arena."""
@@ -219,12 +252,13 @@ class DMRG:
                                                                                 sb.super.dim)
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        psi = ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
+        psi = self._initial(sb, guess)
+        psi0 = self._shadow_start(psi)
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
                                    max_outer=self.lanczos_budget[1], prec=self.prec,
                                    refine_iters=self.refine)
         self.timers["lanczos"] += time.perf_counter() - t0
-        self._shadow(sb, plan, tol, E)
+        self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
         for side, idx, ident in (("sys", 0, identity_sys), ("env", 1, identity_env)):
@@ -235,10 +269,12 @@ class DMRG:
             t0 = time.perf_counter()
             rp = ham.renormalize_plan(blk, site, to_new, keep, U, order, pairs=pairs, resolver=step.offset)
             out[side] = resident.renormalize(ctx, step, rp)
+            out[side + "_trun"] = wavetrans.Trunc(sb.sys_bs if idx == 0 else sb.env_bs, U, order)
             self.timers["renorm"] += time.perf_counter() - t0
             out[side + "_err"], out[side + "_kept"] = err, ns
         plan.close()
-        psi.close()
+        out["psi"] = psi
+        out["bases"] = SimpleNamespace(super=sb.super, sys_bs=sb.sys_bs, env_bs=sb.env_bs)
         step.close()
         self.nstep += 1
         return out
@@ -291,38 +327,67 @@ class DMRG:
                           identity_sys=ident_s, identity_env=ident_e, partners=partners)
             sys, env = r["sys"], r["env"]
             self.sys_blocks[l + 1], self.env_blocks[l + 1] = sys, env
+            self.sys_trun[l + 1], self.env_trun[l + 1] = r["sys_trun"], r["env_trun"]
+            r["psi"].close()
             self.history.append(("warmup", l, r["E"], r.get("sys_err", 0.0), r.get("sys_kept", 0), r["dim"]))
             self._say(f"warm-up l={l} N_e={ntot} E={r['E']:.12f} dim={r['dim']} mv={r['matvecs']}")
             label += 1
 
     # ---- finite sweeps -------------------------------------------------
-    def _sweep_step_sys(self, l, tol, kept_max, trun_err):
+    def _sweep_step_sys(self, l, tol, kept_max, trun_err, guess=None):
         N = self.N
         sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
-        r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err)
+        r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err, guess=guess)
         self.sys_blocks[l + 1] = r["sys"]
+        self.sys_trun[l + 1] = r["sys_trun"]
         return r
 
-    def _sweep_step_env(self, l, tol, kept_max, trun_err):
+    def _sweep_step_env(self, l, tol, kept_max, trun_err, guess=None):
         N = self.N
         sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
-        r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "env", kept_max, trun_err)
+        r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "env", kept_max, trun_err, guess=guess)
         self.env_blocks[N - l - 1] = r["env"]
+        self.env_trun[N - l - 1] = r["env_trun"]
         return r
+
+    def _guess(self, direction, prev, sys_trun, env_trun):
+        """Lanczos start for the next step of a half-sweep, from the previous
+        step's ground state (None: seeded random start)."""
+        if not self.wave_guess or prev is None or sys_trun is None or env_trun is None:
+            return None
+        psi_old, bases_old = prev
+        ctx = self.ctx
+        return lambda sb: wavetrans.transform(ctx, direction, psi_old, bases_old, sb, sys_trun, env_trun)
 
     def left_to_right(self, l_from, l_to, tol, kept_max, trun_err, tag):
+        N = self.N
+        prev = None
         for l in range(l_from, l_to + 1):
-            r = self._sweep_step_sys(l, tol, kept_max, trun_err)
+            guess = self._guess("LR", prev, self.sys_trun.get(l), self.env_trun.get(N - l - 1))
+            r = self._sweep_step_sys(l, tol, kept_max, trun_err, guess)
+            if prev is not None:
+                prev[0].close()
+            prev = (r["psi"], r["bases"])
             self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
             self._say(f"{tag} L->R l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
                       f"dim={r['dim']} mv={r['matvecs']}")
+        if prev is not None:
+            prev[0].close()
 
     def right_to_left(self, l_from, l_to, tol, kept_max, trun_err, tag):
+        N = self.N
+        prev = None
         for l in range(l_from, l_to - 1, -1):
-            r = self._sweep_step_env(l, tol, kept_max, trun_err)
+            guess = self._guess("RL", prev, self.sys_trun.get(l + 1), self.env_trun.get(N - l - 2))
+            r = self._sweep_step_env(l, tol, kept_max, trun_err, guess)
+            if prev is not None:
+                prev[0].close()
+            prev = (r["psi"], r["bases"])
             self.history.append((tag, l, r["E"], r["env_err"], r["env_kept"], r["dim"]))
             self._say(f"{tag} R->L l={l} E={r['E']:.12f} err={r['env_err']:.3e} kept={r['env_kept']} "
                       f"dim={r['dim']} mv={r['matvecs']}")
+        if prev is not None:
+            prev[0].close()
 
     def run(self):
         """Get_Ground_State (main.f90:114-209) without the final measurement
