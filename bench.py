@@ -65,11 +65,12 @@ class PhysicsWorkload:
     """Centre superblock of a YC cylinder at M multiplets: the real term
     list and sector structure of one DMRG step, seeded random blocks."""
 
-    def __init__(self, Lx, Ny, M, seed=7):
+    def __init__(self, Lx, Ny, M, t2=0.2, seed=7):
         from tsc_tri_t1t2j1j2_amd import hamiltonian as ham
         from tsc_tri_t1t2j1j2_amd import workloads
         t0 = time.perf_counter()
-        sysb, envb, s1, s2, bonds, ntot = workloads.centre_superblock(Lx, Ny, M, seed=seed)
+        sysb, envb, s1, s2, bonds, ntot = workloads.centre_superblock(Lx, Ny, M, t2=t2, seed=seed)
+        self.t2 = t2
         self.blocks = (sysb, envb, s1, s2, bonds, ntot)
         self.Lx, self.Ny, self.M = Lx, Ny, M
         self.sb = ham.Superblock(sysb, envb, s1, s2, bonds, ntot, 0)
@@ -100,10 +101,10 @@ class PhysicsWorkload:
 def build_workload(name):
     from tsc_tri_t1t2j1j2_amd.synthetic import SyntheticSuperblock
     if name == "yc6-m6000":
-        wl = PhysicsWorkload(24, 6, 6000)
+        wl = PhysicsWorkload(24, 6, 6000, t2=0.3)
         desc = {"workload": "YC6 6x24 (144 sites, 132 electrons, S=0) centre superblock at M=6000 SU(2) "
                             "multiplets: real t1-t2-J1-J2 term list and sector structure, seeded random blocks",
-                "model": "t1=1 t2=0.2 J1=0.4 J2=0.016", "psi_multiplet_dim": wl.psi_len,
+                "model": "t1=1 t2=0.3 J1=0.4 J2=0.036", "psi_multiplet_dim": wl.psi_len,
                 "sectors": wl.sb.super.num, "plan_build_s": round(wl.plan_seconds, 2)}
         return wl, desc
     if name == "synthetic400":
@@ -215,7 +216,7 @@ def full_step(ctx, wl, prec, args):
     sysb, envb, s1, s2, bonds, ntot = wl.blocks
     out = {}
     for name, p, rf in (("f64", capi.PREC_F64, 0), ("mixed", prec, args.refine)):
-        d = DMRG(ctx, wl.Lx, wl.Ny, kept_min=110, kept_max=wl.M, prec=p, refine=rf, resident=True,
+        d = DMRG(ctx, wl.Lx, wl.Ny, t2=wl.t2, kept_min=110, kept_max=wl.M, prec=p, refine=rf, resident=True,
                  lanczos_budget=(args.lanczos_iters, 0))
         t0 = time.perf_counter()
         sysd, envd = d._dev(sysb), d._dev(envb)
@@ -238,8 +239,43 @@ def full_step(ctx, wl, prec, args):
             "sweep_s_f64_estimate": round(s64 * nsteps, 1), "sweep_s_mixed_estimate": round(smx * nsteps, 1),
             "sweep_speedup_mixed_vs_f64": round(s64 / smx, 3),
             "sweep_estimate_note": f"{nsteps} centre-sized steps x one measured full step (step arena and merges, "
-                                   "plan, Lanczos at the fixed budget, rho/truncation, renormalisation); the "
-                                   "wave-function guess is not implemented (random start vector)"}
+                                   "plan, Lanczos at the fixed budget from a random start, rho/truncation, "
+                                   "renormalisation); in a sweep the wave-function guess lowers the H.psi count per "
+                                   "step (measured_dmrg runs whole sweeps at a smaller size)"}
+
+
+def measured_dmrg(ctx, refine, Lx=8, Ny=4, M=1000, sweeps=2):
+    """Whole DMRG runs (warm-up, two half-sweeps, `sweeps` sweeps; resident
+    operators, wave-function guess) on YC4 Lx x 4 at kept_max M, FP64 and
+    mixed: measured wall time, per-phase times and H.psi counts. The mixed
+    run solves unguessed starts in BF16x3 + FP64 refinement and guessed ones
+    in FP64 (a guessed start converges in a few iterations)."""
+    from tsc_tri_t1t2j1j2_amd import capi
+    from tsc_tri_t1t2j1j2_amd.dmrg import DMRG
+    out = {}
+    for name, prec, rf in (("f64", capi.PREC_F64, 0), ("mixed", capi.PREC_BF16X3, refine)):
+        mv = []
+        t0 = time.perf_counter()
+        d = DMRG(ctx, Lx, Ny, kept_min=110, kept_max=M, sweeps=sweeps, lanzero=1e-10, prec=prec, refine=rf,
+                 guessed_f64=True)
+        orig = d.step
+
+        def step(*a, **k):
+            r = orig(*a, **k)
+            mv.append(r["matvecs"])
+            return r
+
+        d.step = step
+        E = d.run()
+        el = time.perf_counter() - t0
+        out[name] = {"wall_s": round(el, 2), "E_per_site": E / d.N, "steps": d.nstep, "guessed_steps": d.guessed,
+                     "hpsi_calls": int(sum(mv)), "parts_s": {k: round(v, 2) for k, v in d.timers.items()}}
+        print(f"# dmrg YC4 {Lx}x{Ny} M={M} {name}: E={E:.12f} wall {el:.1f}s steps {d.nstep} "
+              f"guessed {d.guessed} H.psi {sum(mv)} {out[name]['parts_s']}", file=sys.stderr, flush=True)
+    return {"measured_dmrg": f"YC4 {Lx}x{Ny} ({Lx * Ny} sites), kept_max {M}, warm-up + 2 half-sweeps + {sweeps} "
+                             f"sweeps, lanzero 1e-10, wave-function guess",
+            "measured_dmrg_f64": out["f64"], "measured_dmrg_mixed": out["mixed"],
+            "measured_dmrg_speedup_mixed_vs_f64": round(out["f64"]["wall_s"] / out["mixed"]["wall_s"], 3)}
 
 
 def dmrg_precision(ctx, sweeps, refine):
@@ -372,6 +408,7 @@ def main():
             extra.update(full_step(ctx, sb, prec, args))
     if isinstance(sb, PhysicsWorkload) and not (args.no_lanczos or args.no_dmrg) and prec != capi.PREC_F64:
         extra.update(dmrg_precision(ctx, args.dmrg_sweeps, args.refine))
+        extra.update(measured_dmrg(ctx, args.refine))
     cpu = None
     if not args.no_cpu:
         psi_host = vin.download()

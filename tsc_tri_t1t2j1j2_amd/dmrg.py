@@ -48,7 +48,7 @@ class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
                  prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True,
-                 lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True):
+                 lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True, guessed_f64=False):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -72,6 +72,11 @@ class DMRG:
         # wave_guess: inside a half-sweep, start each Lanczos from the previous
         # ground state carried into the new superblock (wave_trans_LR/RL)
         self.wave_guess = wave_guess
+        # guessed_f64: solve the steps that start from the guess in FP64 even
+        # when prec is a mixed one (a guessed start converges in a few
+        # iterations, which the FP64 refinement would repeat anyway)
+        self.guessed_f64 = guessed_f64
+        self._prec_now = prec
         self.sys_trun, self.env_trun = {}, {}   # block length -> wavetrans.Trunc that built it
         self.seed = seed
         self.native_planner = native_planner
@@ -135,17 +140,20 @@ class DMRG:
         """Lanczos start: the transformed previous ground state when the
         sweep provides one (wave_trans_LR/RL), else the seeded random vector
         (wave_initialize)."""
+        self._prec_now = self.prec
         if guess is not None:
             t0 = time.perf_counter()
             v = guess(sb)
             self.timers["guess"] += time.perf_counter() - t0
             if v is not None:
                 self.guessed += 1
+                if self.guessed_f64:
+                    self._prec_now = capi.PREC_F64
                 return v
         return self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
 
     def _shadow_start(self, psi):
-        if not self.shadow_f64 or self.prec == capi.PREC_F64:
+        if not self.shadow_f64 or self._prec_now == capi.PREC_F64:
             return None
         v = self.ctx.vec(psi.n)
         v.copy_from(psi)
@@ -191,8 +199,8 @@ class DMRG:
         psi = self._initial(sb, guess)
         psi0 = self._shadow_start(psi)
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
-                                   max_outer=self.lanczos_budget[1], prec=self.prec,
-                                   refine_iters=self.refine)
+                                   max_outer=self.lanczos_budget[1], prec=self._prec_now,
+                                   refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
         self.timers["lanczos"] += time.perf_counter() - t0
         self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
@@ -255,8 +263,8 @@ class DMRG:
         psi = self._initial(sb, guess)
         psi0 = self._shadow_start(psi)
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
-                                   max_outer=self.lanczos_budget[1], prec=self.prec,
-                                   refine_iters=self.refine)
+                                   max_outer=self.lanczos_budget[1], prec=self._prec_now,
+                                   refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
         self.timers["lanczos"] += time.perf_counter() - t0
         self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
