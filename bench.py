@@ -33,10 +33,9 @@ HBM_GBS = 8000.0
 # with --pmc FETCH_SIZE and --pmc WRITE_SIZE; each counter is scaled by its
 # reading for the 1 GiB copy in the same run (FETCH_SIZE x 2.0, WRITE_SIZE
 # x 1.0). Per-dispatch KiB: BF16x3 pk_gemm_kernel 14388044 fetched, 617783
-# written; FP64 grouped_kernel 29900490 fetched, 344647 written.
+# written.
 MEASURED_TRAFFIC = {
     ("yc6-m6000", "bf16x3"): (14388044 * 2.0 + 617783 * 1.0) * 1024,
-    ("yc6-m6000", "f64"): (29900490 * 2.0 + 344647 * 1.0) * 1024,
 }
 
 
@@ -227,15 +226,24 @@ def full_step(ctx, wl, prec, args):
         ctx.sync()
         tstep = time.perf_counter() - t0
         tm = {k: round(v, 3) for k, v in d.timers.items()}
+        rfl, rms = d.renorm_device
+        tr_s = d.timers["truncate"]
+        side = {"renorm_uou": {"bound": "mfma", "achieved": round(rfl / (rms * 1e-3) / 1e12, 3) if rms else None,
+                               "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s", "flops": rfl,
+                               "device_ms": round(rms, 3)},
+                "rho_truncate": {"bound": "hbm", "achieved": round(d.trunc_bytes / tr_s / 1e9, 3) if tr_s else None,
+                                 "peak": 8000.0, "unit": "GB/s", "bytes": d.trunc_bytes,
+                                 "wall_ms": round(tr_s * 1e3, 2)}}
         print(f"# full step {name}: {tstep:.2f}s {tm} kept {r['sys_kept']} E={r['E']:.10f} "
               f"(upload of both blocks {tup:.2f}s)", file=sys.stderr, flush=True)
-        out[name] = (tstep, tm, r["sys_kept"], tup)
+        out[name] = (tstep, tm, r["sys_kept"], tup, side)
         del r, d, sysd, envd
     nsteps = 2 * (wl.N - 3)
     s64, smx = out["f64"][0], out["mixed"][0]
     return {"full_step_s_f64": round(s64, 3), "full_step_s_mixed": round(smx, 3),
             "full_step_parts_f64": out["f64"][1], "full_step_parts_mixed": out["mixed"][1],
             "full_step_kept": int(out["mixed"][2]), "block_upload_s": round(out["mixed"][3], 3),
+            "full_step_other_rooflines_f64": out["f64"][4],
             "sweep_s_f64_estimate": round(s64 * nsteps, 1), "sweep_s_mixed_estimate": round(smx * nsteps, 1),
             "sweep_speedup_mixed_vs_f64": round(s64 / smx, 3),
             "sweep_estimate_note": f"{nsteps} centre-sized steps x one measured full step (step arena and merges, "

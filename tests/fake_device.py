@@ -93,6 +93,9 @@ class FakePlan:
     def flops(self):
         return 0.0
 
+    def last_times(self):
+        return 0.0, 0.0
+
     def close(self):
         pass
 

@@ -96,6 +96,11 @@ class DMRG:
         self.hpsi_flops = 0.0
         self.timers = {"plan": 0.0, "guess": 0.0, "lanczos": 0.0, "truncate": 0.0, "renorm": 0.0}
         self.guessed = 0
+        # U^dagger O U contractions: algorithmic flops, device ms; rho and
+        # truncation: algorithmic HBM bytes (psi read, rho written and
+        # factorised in place, U copied out)
+        self.renorm_device = [0.0, 0.0]
+        self.trunc_bytes = 0.0
 
     # ---- helpers ---------------------------------------------------------
     def _say(self, msg):
@@ -126,6 +131,9 @@ class DMRG:
         t0 = time.perf_counter()
         res = capi.rho_truncate(self.ctx, psi_vec, sd, ed, twos, side, kept_min, kept_max, trun_err)
         self.timers["truncate"] += time.perf_counter() - t0
+        dd = np.asarray(sd if side == 0 else ed, np.float64)
+        kk = np.asarray(res["kept"], np.float64)
+        self.trunc_bytes += 16.0 * (psi_vec.n + 3.0 * float(np.sum(dd * dd)) + float(np.sum(dd * kk)))
         U, order = {}, []
         for x, u, k in zip(xs, res["U"], res["kept"]):
             if k > 0:
@@ -277,6 +285,9 @@ class DMRG:
             t0 = time.perf_counter()
             rp = ham.renormalize_plan(blk, site, to_new, keep, U, order, pairs=pairs, resolver=step.offset)
             out[side] = resident.renormalize(ctx, step, rp)
+            fl, ms = getattr(out[side], "renorm_stats", (0.0, 0.0))
+            self.renorm_device[0] += fl
+            self.renorm_device[1] += ms
             out[side + "_trun"] = wavetrans.Trunc(sb.sys_bs if idx == 0 else sb.env_bs, U, order)
             self.timers["renorm"] += time.perf_counter() - t0
             out[side + "_err"], out[side + "_kept"] = err, ns

@@ -256,9 +256,12 @@ def renormalize(ctx, step, rp, prec=capi.PREC_F64):
     out = ctx.vec(rp.out_len)
     plan.apply(uin, out, prec)
     ctx.sync()
+    # algorithmic flops and device time of the contraction (both stages)
+    stats = (plan.flops(), sum(plan.last_times()))
     plan.close()
     uin.close()
     arena = DevArena(out)
     model = renormalize_result(rp, None, arena=arena)
     model.arena = arena
+    model.renorm_stats = stats
     return model
