@@ -411,28 +411,7 @@ def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order, pairs=No
         parts.append(u.ravel(order="F"))
         off += u.size
     uin = np.concatenate(parts) if parts else np.zeros(1, np.complex128)
-    pb = PlanBuilder(resolver=resolver)
-    out_off = 0
-    windows = {}
-    for opkey, tl in trans.items():
-        for (x, p, d, x2, p2, d2, f, mat) in tl:
-            if x not in new_index or x2 not in new_index:
-                continue
-            kx, kx2 = U[x].shape[1], U[x2].shape[1]
-            dx, dx2 = basis.sectors[x][2], basis.sectors[x2][2]
-            wk = (opkey, x2, x)
-            if wk not in windows:
-                windows[wk] = (pb.out_window(wk, out_off, kx2, kx2, kx), out_off)
-                out_off += kx2 * kx
-            wid = windows[wk][0]
-            ul = Operand(capi.BUF_IN, uoff[x2] + p2, dx2, capi.OP_C)       # U_x2[rows]^H
-            ur = Operand(capi.BUF_IN, uoff[x] + p, dx, capi.OP_N)          # U_x[rows]
-            if mat is None:
-                pb.add(wid, ul, ur, d, f)
-            else:
-                mo, mld = pb.op(mat[0], mat[1])
-                Z = pb.intermediate((mat[0], x, p), Operand(capi.BUF_OPS, mo, mld, mat[2]), ur, d2, kx, d)
-                pb.add(wid, ul, Z, d2, f)
+    pb, windows, out_off = _renorm_tasks(trans, basis, U, order, uoff, resolver)
     # diagonal H blocks no term reaches stay zero (untouched output windows)
     for x in order:
         wk = ("H", x, x)
@@ -446,6 +425,147 @@ def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order, pairs=No
     rp.blk, rp.new_site, rp.keep_sites = blk, new_site, keep_sites
     rp.keepalive = (pairs, trans)
     return rp
+
+
+class ArrayPlan:
+    """Task arrays built directly (the PlanBuilder interface the callers use:
+    arrays, arena_array, mid_len, build, build_dev)."""
+
+    def __init__(self, c1, o1, c2, o2, mid_len, arena):
+        self._arr = (c1, o1, c2, o2)
+        self.mid_len = int(mid_len)
+        self._arena = arena
+
+    def arrays(self):
+        return self._arr
+
+    def arena_array(self):
+        return self._arena if self._arena is not None else np.zeros(1, np.complex128)
+
+    def build(self, ctx, in_len, out_len):
+        c1, o1, c2, o2 = self._arr
+        ops = self.arena_array()
+        return capi.Plan(ctx, ops, ops.size, self.mid_len, c1, o1, c2, o2, int(in_len), int(out_len))
+
+    def build_dev(self, ctx, ops_vec, in_len, out_len):
+        c1, o1, c2, o2 = self._arr
+        return capi.Plan.create_dev(ctx, ops_vec, self.mid_len, c1, o1, c2, o2, int(in_len), int(out_len))
+
+
+def _renorm_tasks(trans, basis, U, order, uoff, resolver):
+    """Task arrays of U_x2^H . T . U_x for every transition, vectorised.
+
+    Output windows (opkey, x2, x) are laid out in order of first appearance,
+    as are the stage-1 intermediates T U_x[rows] (one per (matrix, x, p)).
+    Returns (plan, windows {(opkey, x2, x): (window id, offset)}, length)."""
+    nsec = len(basis.sectors)
+    kept = np.full(nsec, -1, np.int64)
+    for x in order:
+        kept[x] = U[x].shape[1]
+    dims = np.array([sec[2] for sec in basis.sectors], np.int64)
+    uo = np.zeros(nsec, np.int64)
+    for x, o in uoff.items():
+        uo[x] = o
+    # flatten the transitions; matrix keys to ids, stored matrices to offsets
+    opkeys = list(trans.keys())
+    rows, mkeys, mats, mid_of = [], {}, [], []
+    for ki, opkey in enumerate(opkeys):
+        for (x, p, d, x2, p2, d2, f, mat) in trans[opkey]:
+            if mat is None:
+                mi = -1
+            else:
+                mi = mkeys.get(mat[0])
+                if mi is None:
+                    mi = mkeys[mat[0]] = len(mats)
+                    mats.append(mat)
+            rows.append((ki, x, p, d, x2, p2, d2, mi, f))
+    if not rows:
+        z = np.zeros(0, capi.CONTRIB_DTYPE)
+        return ArrayPlan(z, np.zeros(0, capi.OUTBLOCK_DTYPE), z, np.zeros(0, capi.OUTBLOCK_DTYPE), 0, None), {}, 0
+    R = np.array([r[:8] for r in rows], np.int64)
+    F = np.array([r[8] for r in rows], np.complex128)
+    ki, x, p, d, x2, p2, d2, mi = R.T
+    sel = (kept[x] >= 0) & (kept[x2] >= 0) & (F != 0)
+    ki, x, p, d, x2, p2, d2, mi, F = ki[sel], x[sel], p[sel], d[sel], x2[sel], p2[sel], d2[sel], mi[sel], F[sel]
+    kx, kx2, dx, dx2 = kept[x], kept[x2], dims[x], dims[x2]
+    # windows in first-appearance order
+    wkey = (ki * nsec + x2) * nsec + x
+    uniq, first, inv = np.unique(wkey, return_index=True, return_inverse=True)
+    rank = np.empty(len(uniq), np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(len(uniq))
+    wid = rank[inv]
+    nw = len(uniq)
+    wx = np.empty(nw, np.int64)
+    wx2 = np.empty(nw, np.int64)
+    wk = np.empty(nw, np.int64)
+    wx[wid], wx2[wid], wk[wid] = x, x2, ki
+    wsize = kept[wx2] * kept[wx]
+    woff = np.concatenate(([0], np.cumsum(wsize)[:-1])) if nw else np.zeros(0, np.int64)
+    windows = {(opkeys[int(wk[w])], int(wx2[w]), int(wx[w])): (int(w), int(woff[w])) for w in range(nw)}
+    out_len = int(wsize.sum()) if nw else 0
+    # matrix operands: arena offsets (resolver) or a plan-owned host arena
+    mo = np.zeros(max(len(mats), 1), np.int64)
+    mld = np.ones(max(len(mats), 1), np.int64)
+    mop = np.zeros(max(len(mats), 1), np.int64)
+    arena, alen = [], 0
+    for i, (key, stored, opc) in enumerate(mats):
+        if resolver is not None:
+            o, l = resolver(stored)
+        else:
+            m = np.asarray(stored, np.complex128)
+            o, l = alen, max(m.shape[0], 1)
+            arena.append(m.ravel(order="F"))
+            alen += m.size
+        mo[i], mld[i], mop[i] = o, l, opc
+    # stage-1 intermediates T U_x[rows] per (matrix, x, p), first appearance
+    hm = mi >= 0
+    ikey = (mi[hm] * nsec + x[hm]) * (int(p.max()) + 1 if len(p) else 1) + p[hm]
+    iu, ifirst, iinv = np.unique(ikey, return_index=True, return_inverse=True)
+    irank = np.empty(len(iu), np.int64)
+    irank[np.argsort(ifirst, kind="stable")] = np.arange(len(iu))
+    iid = irank[iinv]
+    ni = len(iu)
+    hidx = np.nonzero(hm)[0]
+    rep = np.empty(ni, np.int64)
+    rep[iid] = hidx                       # a transition that uses each intermediate
+    isize = d2[rep] * kx[rep]
+    ioff = np.concatenate(([0], np.cumsum(isize)[:-1])) if ni else np.zeros(0, np.int64)
+    mid_len = int(isize.sum()) if ni else 0
+    c1 = np.zeros(ni, capi.CONTRIB_DTYPE)
+    o1 = np.zeros(ni, capi.OUTBLOCK_DTYPE)
+    if ni:
+        r = rep
+        c1["a_off"], c1["lda"], c1["a_buf"], c1["a_op"] = mo[mi[r]], mld[mi[r]], capi.BUF_OPS, mop[mi[r]]
+        c1["b_off"], c1["ldb"], c1["b_buf"], c1["b_op"] = uo[x[r]] + p[r], np.maximum(dx[r], 1), capi.BUF_IN, capi.OP_N
+        c1["k"], c1["coef_re"] = d[r], 1.0
+        o1["c_off"], o1["ldc"], o1["m"], o1["n"] = ioff, np.maximum(d2[r], 1), d2[r], kx[r]
+        o1["c_buf"] = capi.BUF_MID
+        o1["cbeg"] = np.arange(ni)
+        o1["cend"] = np.arange(ni) + 1
+    # stage 2, grouped by window (stable: transition order within a window)
+    order2 = np.argsort(wid, kind="stable")
+    c2 = np.zeros(len(order2), capi.CONTRIB_DTYPE)
+    t = order2
+    c2["a_off"], c2["lda"], c2["a_buf"], c2["a_op"] = uo[x2[t]] + p2[t], np.maximum(dx2[t], 1), capi.BUF_IN, capi.OP_C
+    mid_row = np.full(len(x), -1, np.int64)
+    mid_row[hidx] = ioff[iid]
+    withm = hm[t]
+    c2["b_off"] = np.where(withm, mid_row[t], uo[x[t]] + p[t])
+    c2["ldb"] = np.where(withm, np.maximum(d2[t], 1), np.maximum(dx[t], 1))
+    c2["b_buf"] = np.where(withm, capi.BUF_MID, capi.BUF_IN)
+    c2["b_op"] = capi.OP_N
+    c2["k"] = np.where(withm, d2[t], d[t])
+    c2["coef_re"], c2["coef_im"] = F[t].real, F[t].imag
+    o2 = np.zeros(nw, capi.OUTBLOCK_DTYPE)
+    if nw:
+        counts = np.bincount(wid, minlength=nw)
+        ends = np.cumsum(counts)
+        o2["c_off"], o2["ldc"], o2["m"], o2["n"] = woff, np.maximum(kept[wx2], 1), kept[wx2], kept[wx]
+        o2["c_buf"] = capi.BUF_OUT
+        o2["cbeg"] = ends - counts
+        o2["cend"] = ends
+    plan = ArrayPlan(c1, o1, c2, o2, mid_len, np.concatenate(arena) if arena else None)
+    return plan, windows, out_len
 
 
 def renormalize_result(rp, res, arena=None):

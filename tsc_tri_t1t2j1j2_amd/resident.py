@@ -199,24 +199,24 @@ class StepArena:
             self.vec.copy_range(self.base[id(a)], a.vec, 0, a.n)
         self.as_arena = DevArena(self.vec)
         if self.merges:
-            c2, o2 = [], []
+            # one output window per merged operator, one elementwise term per part
+            ob_rows, c_rows = [], []
             for pm in self.merges:
                 rows, cols = pm.shape
-                ob = np.zeros((), capi.OUTBLOCK_DTYPE)
-                ob["c_off"], ob["ldc"], ob["m"], ob["n"] = self._merge_off[id(pm)], max(rows, 1), rows, cols
-                ob["c_buf"], ob["cbeg"] = capi.BUF_OUT, len(c2)
+                cbeg = len(c_rows)
                 for w, m in pm.parts:
-                    e = np.zeros((), capi.CONTRIB_DTYPE)
-                    e["a_off"], e["lda"], e["a_buf"], e["a_op"] = self.base[id(m.arena)] + m.off, max(rows, 1), \
-                        capi.BUF_OPS, capi.OP_N
-                    e["b_op"] = capi.OP_I
-                    e["coef_re"] = w
-                    c2.append(e)
-                ob["cend"] = len(c2)
-                o2.append(ob)
+                    c_rows.append((self.base[id(m.arena)] + m.off, max(rows, 1), w))
+                ob_rows.append((self._merge_off[id(pm)], max(rows, 1), rows, cols, cbeg, len(c_rows)))
+            c2 = np.zeros(len(c_rows), capi.CONTRIB_DTYPE)
+            ca = np.array(c_rows, dtype=np.float64)
+            c2["a_off"], c2["lda"], c2["coef_re"] = ca[:, 0].astype(np.int64), ca[:, 1].astype(np.int32), ca[:, 2]
+            c2["a_buf"], c2["a_op"], c2["b_op"] = capi.BUF_OPS, capi.OP_N, capi.OP_I
+            o2 = np.zeros(len(ob_rows), capi.OUTBLOCK_DTYPE)
+            oa = np.array(ob_rows, dtype=np.int64)
+            o2["c_off"], o2["ldc"], o2["m"], o2["n"], o2["cbeg"], o2["cend"] = oa.T
+            o2["c_buf"] = capi.BUF_OUT
             plan = capi.Plan.create_dev(ctx, self.vec, 0, np.zeros(0, capi.CONTRIB_DTYPE),
-                                        np.zeros(0, capi.OUTBLOCK_DTYPE), np.array(c2, capi.CONTRIB_DTYPE),
-                                        np.array(o2, capi.OUTBLOCK_DTYPE), 0, self.vec.n)
+                                        np.zeros(0, capi.OUTBLOCK_DTYPE), c2, o2, 0, self.vec.n)
             dummy = ctx.vec(1)
             plan.apply(dummy, self.vec, capi.PREC_F64)
             ctx.sync()
