@@ -30,7 +30,13 @@ def main(argv=None):
     ap.add_argument("--lanzero", type=float, default=1e-10)
     ap.add_argument("--trun-err", type=float, default=1e-15)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "bf16x3"])
-    ap.add_argument("--refine", type=int, default=20)
+    ap.add_argument("--refine", type=int, default=4)
+    ap.add_argument("--no-guess", action="store_true", help="start every Lanczos from a random vector")
+    ap.add_argument("--guessed-f64", action="store_true",
+                    help="with a mixed precision, solve the steps that start from the guess in FP64")
+    ap.add_argument("--shadow-f64", action="store_true",
+                    help="with a mixed precision, also solve every superblock in FP64 (same start) and report "
+                         "the per-step energy difference")
     ap.add_argument("--seed", type=int, default=1234)
     a = ap.parse_args(argv)
     ctx = capi.Context(0)
@@ -42,7 +48,8 @@ def main(argv=None):
     prec = capi.PREC_NAMES[a.prec]
     d = DMRG(ctx, a.Lx, a.Ny, t1=a.t1, t2=a.t2, J1=a.J1, J2=a.J2, delta=a.delta, kept_min=a.kept_min,
              kept_max=a.kept_max, kept_maxf=a.kept_maxf, sweeps=a.sweeps, lanzero=a.lanzero, trun_err=a.trun_err,
-             prec=prec, refine=a.refine if prec != capi.PREC_F64 else 0, seed=a.seed, log=log)
+             prec=prec, refine=a.refine if prec != capi.PREC_F64 else 0, seed=a.seed, log=log,
+             wave_guess=not a.no_guess, guessed_f64=a.guessed_f64, shadow_f64=a.shadow_f64)
     E = d.run()
     wall = time.perf_counter() - t0
     sweep_rows = [h for h in d.history if h[0].startswith("sweep")]
@@ -53,8 +60,12 @@ def main(argv=None):
         "E_min_last_sweep": min(h[2] for h in last) if last else E,
         "max_trun_err_last_sweep": max(h[3] for h in last) if last else None,
         "wall_s": round(wall, 2), "timers_s": {k: round(v, 2) for k, v in d.timers.items()},
-        "hpsi_flops_total": d.hpsi_flops, "steps": len(d.history),
+        "hpsi_flops_total": d.hpsi_flops, "steps": len(d.history), "guessed_steps": d.guessed,
     }
+    if d.shadow:
+        de = [abs(em - e64) / d.N for (_, em, e64) in d.shadow]
+        out["same_path_max_dE_per_site"] = max(de)
+        out["same_path_last_dE_per_site"] = de[-1]
     print(json.dumps(out), flush=True)
 
 
