@@ -33,6 +33,9 @@ class FakeVec:
     def norm(self):
         return float(np.linalg.norm(self.x))
 
+    def dot_re(self, other):
+        return float(np.vdot(self.x, other.x).real)
+
     def axpby(self, a, x, b):
         self.x[:] = a * x.x + b * self.x
 

@@ -378,7 +378,9 @@ class DMRG:
         ctx = self.ctx
         return lambda sb: wavetrans.transform(ctx, direction, psi_old, bases_old, sb, sys_trun, env_trun)
 
-    def left_to_right(self, l_from, l_to, tol, kept_max, trun_err, tag):
+    def left_to_right(self, l_from, l_to, tol, kept_max, trun_err, tag, keep=False):
+        """keep: return the last step's (ground state, bases) instead of
+        releasing the state."""
         N = self.N
         prev = None
         for l in range(l_from, l_to + 1):
@@ -390,6 +392,8 @@ class DMRG:
             self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
             self._say(f"{tag} L->R l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
                       f"dim={r['dim']} mv={r['matvecs']}")
+        if keep:
+            return prev
         if prev is not None:
             prev[0].close()
 
@@ -426,4 +430,43 @@ class DMRG:
             tol = lanerr * (0.5 if it >= 5 else 1.0)
             self.left_to_right(1, N - 3, tol, kept_max, self.trun_err, f"sweep{it}")
             self.right_to_left(N - 3, 1, tol, kept_max, self.trun_err, f"sweep{it}")
+        self._final = (tol, kept_max, self.trun_err if self.sweeps else terr)
         return self.history[-1][2]
+
+    def measure(self, i0=None, js=None, kinds=("SS", "nn", "cc"), l_end=None, density=True, pair_bonds=(1, 3, 5)):
+        """After run(): the final left-to-right half-sweep to l_end (5N/6 - 1,
+        main.f90:205-206) and the measurement of its last ground state
+        (Measurement, main.f90:213-354): densities of every site, the
+        correlators `kinds` of site i0 with the sites js (default: the
+        reference's site xi1 and the sites of its leg) and, for each bond
+        type of pair_bonds, the singlet pair correlators the reference
+        writes to Sc_cor_y / _x / _xy (main.f90:259-263)."""
+        from . import measure as meas
+        N = self.N
+        l_end = 5 * N // 6 - 1 if l_end is None else int(l_end)
+        if not 1 <= l_end <= N - 3:
+            raise ValueError(f"final half-sweep end {l_end} outside 1..{N - 3}")
+        d_i0, d_js = meas.reference_sites(self.Lx, self.Ny)
+        i0 = d_i0 if i0 is None else int(i0)
+        js = [j for j in (d_js if js is None else js) if j > i0]
+        tol, kept_max, terr = getattr(self, "_final", (self.lanzero, self.kept_maxf, self.trun_err))
+        psi, bases = self.left_to_right(1, l_end, tol, kept_max, terr, "final", keep=True)
+        t0 = time.perf_counter()
+        try:
+            out = meas.measure_state(self.ctx, psi, bases, l_end + 1, N, self.sys_trun, self.env_trun, i0, js,
+                                     kinds=kinds, density=density)
+            out["pairs"] = {}
+            for ij1 in pair_bonds:
+                bond, partners = meas.reference_pair_partners(self.Lx, self.Ny, l_end + 1, ij1)
+                vals = {}
+                if partners:
+                    got = meas.measure_pairs(self.ctx, psi, bases, l_end + 1, N, self.sys_trun, self.env_trun, bond,
+                                             [p for (_, _, p) in partners])
+                    vals = {(sid3, t): got[tuple(sorted(p))] for (sid3, t, p) in partners}
+                out["pairs"][ij1] = {"bond": bond, "values": vals}
+        finally:
+            psi.close()
+        out["energy"] = self.history[-1][2]
+        out["i0"], out["js"], out["l_end"] = i0, js, l_end
+        self.timers["measure"] = time.perf_counter() - t0
+        return out

@@ -308,47 +308,54 @@ class Superblock:
         return pb
 
     def _cross(self, pb, win, TL, TR, g, cut, k2, y_odd):
-        L, R, SB = self.sys_bs, self.env_bs, self.super
-        S2 = SB.s2tot
-        byx, byy = {}, {}
-        for t in TL:
-            byx.setdefault(t[0], []).append(t)
-        for t in TR:
-            byy.setdefault(t[0], []).append(t)
-        for i, (x, y, dx, dy) in enumerate(SB.sub):
-            tl, tr = byx.get(x), byy.get(y)
-            if not tl or not tr:
-                continue
-            off = SB.offset[i]
-            Jx, Jy = L.sectors[x][1], R.sectors[y][1]
-            fsgn = -1.0 if (y_odd and L.sectors[x][0] % 2) else 1.0
-            for (_, p, d, x2, p2, d2, fL, Lm) in tl:
-                for (_, q, e, y2, q2, e2, fR, Rm) in tr:
-                    i2 = SB.index.get((x2, y2))
-                    if i2 is None:
-                        continue
-                    f = g * fL * fR
-                    if cut:
-                        f *= su2.cut_pair(L.sectors[x2][1], Jx, R.sectors[y2][1], Jy, S2, k2) * fsgn
-                    if f == 0.0:
-                        continue
-                    out = win[(i2, p2, q2)]
-                    src = Operand(capi.BUF_IN, off + p + q * dx, dx, capi.OP_N)
-                    if Lm is not None:
-                        lo, lld = pb.op(Lm[0], Lm[1])
-                        lop = Operand(capi.BUF_OPS, lo, lld, Lm[2])
-                    if Rm is not None:
-                        ro, rld = pb.op(Rm[0], Rm[1])
-                        rop = Operand(capi.BUF_OPS, ro, rld, capi.OP_T if Rm[2] == capi.OP_N else capi.OP_CONJ)
-                    if Lm is not None and Rm is not None:
-                        Z = pb.intermediate((Lm[0], i, p, q), lop, src, d2, e, d)
-                        pb.add(out, Z, rop, e, f)
-                    elif Lm is not None:
-                        pb.add(out, lop, src, d, f)
-                    elif Rm is not None:
-                        pb.add(out, src, rop, e, f)
-                    else:
-                        pb.add(out, src, IDENT, 0, f)
+        cross_terms(pb, win, self.sys_bs, self.env_bs, self.super, TL, TR, g, cut, k2, y_odd)
+
+
+def cross_terms(pb, win, L, R, SB, TL, TR, g, cut, k2, y_odd):
+    """Contributions of an L-side x R-side product to the windows win[(i, p,
+    q)] of psi: TL / TR are transitions of the two sides (identity, scalar or
+    rank-k2 parts); cut couples two rank-k2 parts into the total spin;
+    y_odd: the R part is fermionic (sign of the L sector's particles)."""
+    S2 = SB.s2tot
+    byx, byy = {}, {}
+    for t in TL:
+        byx.setdefault(t[0], []).append(t)
+    for t in TR:
+        byy.setdefault(t[0], []).append(t)
+    for i, (x, y, dx, dy) in enumerate(SB.sub):
+        tl, tr = byx.get(x), byy.get(y)
+        if not tl or not tr:
+            continue
+        off = SB.offset[i]
+        Jx, Jy = L.sectors[x][1], R.sectors[y][1]
+        fsgn = -1.0 if (y_odd and L.sectors[x][0] % 2) else 1.0
+        for (_, p, d, x2, p2, d2, fL, Lm) in tl:
+            for (_, q, e, y2, q2, e2, fR, Rm) in tr:
+                i2 = SB.index.get((x2, y2))
+                if i2 is None:
+                    continue
+                f = g * fL * fR
+                if cut:
+                    f *= su2.cut_pair(L.sectors[x2][1], Jx, R.sectors[y2][1], Jy, S2, k2) * fsgn
+                if f == 0.0:
+                    continue
+                out = win[(i2, p2, q2)]
+                src = Operand(capi.BUF_IN, off + p + q * dx, dx, capi.OP_N)
+                if Lm is not None:
+                    lo, lld = pb.op(Lm[0], Lm[1])
+                    lop = Operand(capi.BUF_OPS, lo, lld, Lm[2])
+                if Rm is not None:
+                    ro, rld = pb.op(Rm[0], Rm[1])
+                    rop = Operand(capi.BUF_OPS, ro, rld, capi.OP_T if Rm[2] == capi.OP_N else capi.OP_CONJ)
+                if Lm is not None and Rm is not None:
+                    Z = pb.intermediate((Lm[0], i, p, q), lop, src, d2, e, d)
+                    pb.add(out, Z, rop, e, f)
+                elif Lm is not None:
+                    pb.add(out, lop, src, d, f)
+                elif Rm is not None:
+                    pb.add(out, src, rop, e, f)
+                else:
+                    pb.add(out, src, IDENT, 0, f)
 
 
 # ---------------------------------------------------------------------

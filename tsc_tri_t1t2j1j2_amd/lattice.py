@@ -55,3 +55,22 @@ def target_electrons(N, delta=12):
     doping 1/delta, rounded to an even number."""
     q = N // 2 - (N + 2 * delta - 1) // (2 * delta)
     return 2 * q
+
+
+def neighbour(i, typ, Lx, Ny):
+    """neib(i, typ) of the reference for the odd bond types (dmrg_sub.f90:
+    334-404): 1 the y neighbour (iy+1, periodic; none from the last leg of a
+    2-leg cylinder), 3 the x neighbour, 5 the (x+1, y-1) neighbour; 0 when
+    the cylinder ends."""
+    ix, iy = (i - 1) // Ny + 1, (i - 1) % Ny + 1
+    if typ == 1:
+        if Ny == 2 and iy == Ny:
+            return 0
+        return iy % Ny + 1 + (ix - 1) * Ny
+    if ix + 1 > Lx:
+        return 0
+    if typ == 3:
+        return iy + ix * Ny
+    if typ == 5:
+        return (iy - 2) % Ny + 1 + ix * Ny
+    raise ValueError(f"bond type {typ} is not one of 1, 3, 5")

@@ -38,6 +38,9 @@ def main(argv=None):
                     help="with a mixed precision, also solve every superblock in FP64 (same start) and report "
                          "the per-step energy difference")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--measure", default=None, metavar="DIR",
+                    help="after the sweeps, sweep left to right to 5N/6 and write the reference's measurement "
+                         "files (densities, spin / hopping / density correlators, singlet pair correlators) to DIR")
     a = ap.parse_args(argv)
     ctx = capi.Context(0)
     t0 = time.perf_counter()
@@ -66,6 +69,15 @@ def main(argv=None):
         de = [abs(em - e64) / d.N for (_, em, e64) in d.shadow]
         out["same_path_max_dE_per_site"] = max(de)
         out["same_path_last_dE_per_site"] = de[-1]
+    if a.measure:
+        from . import measure as meas
+        t1 = time.perf_counter()
+        res = d.measure()
+        out["measure_s"] = round(time.perf_counter() - t1, 2)
+        out["measure_stats"] = res["stats"]
+        out["measure_files"] = meas.write_reference_files(res, a.measure, a.Lx, a.Ny, d.J2, d.kept_min,
+                                                          d.kept_maxf, d.Ne)
+        out["E_measured_state"] = res["energy"]
     print(json.dumps(out), flush=True)
 
 
