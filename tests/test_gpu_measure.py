@@ -73,3 +73,20 @@ def test_device_measurement_sum_rules_under_truncation(ctx):
     assert abs(sum(out["nn"].values()) - (Ne - 1) * n[1]) < 1e-9
     assert all(0.0 < x < 1.0 for x in n.values())
     assert set(out["pairs"]) == {1, 3, 5}
+
+
+def test_device_checkpoint_and_measurement_replay(ctx, tmp_path):
+    """The per-length files written during a device run reproduce its
+    measurement when replayed (restart1 = 4) and restart it (restart1 = 2)."""
+    d = DMRG(ctx, 4, 3, Ne=10, kept_min=20, kept_max=40, sweeps=1, lanzero=1e-10, trun_err=1e-12)
+    e = d.run(checkpoint=str(tmp_path))
+    out = d.measure()
+    rep = meas.measure_from_disk(ctx, str(tmp_path), 4, 3)
+    for kind in ("SS", "nn", "cc"):
+        for key, v in out[kind].items():
+            assert abs(rep[kind][key] - v) < 1e-12, (kind, key)
+    for j, v in out["density"].items():
+        assert abs(rep["density"][j] - v) < 1e-12
+    again = DMRG(ctx, 4, 3, Ne=10, kept_min=20, kept_max=40, sweeps=1, lanzero=1e-10, trun_err=1e-12)
+    e2 = again.run(checkpoint=str(tmp_path), restart=(2, 0, 0))
+    assert abs(e2 - e) < 1e-6, (e2, e)

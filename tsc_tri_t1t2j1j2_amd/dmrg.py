@@ -11,6 +11,8 @@ blocks. Start vectors are seeded uniform random numbers (wave_initialize,
 block.f90:1154-1180, made reproducible); the guess transformation of the
 reference (wave_trans_LR/RL) is not used, each Lanczos starts afresh.
 """
+import json
+import os
 import time
 
 import numpy as np
@@ -101,6 +103,11 @@ class DMRG:
         # factorised in place, U copied out)
         self.renorm_device = [0.0, 0.0]
         self.trunc_bytes = 0.0
+        # checkpoint directory: every new block, its truncation and basis,
+        # and each step's ground state are written there in the reference's
+        # per-length files (persist.py), as model_to_disk / truns_to_disk /
+        # basis_to_disk / wave_to_disk do at each step of the reference
+        self.checkpoint = None
 
     # ---- helpers ---------------------------------------------------------
     def _say(self, msg):
@@ -347,8 +354,12 @@ class DMRG:
             sys, env = r["sys"], r["env"]
             self.sys_blocks[l + 1], self.env_blocks[l + 1] = sys, env
             self.sys_trun[l + 1], self.env_trun[l + 1] = r["sys_trun"], r["env_trun"]
+            self._save_block("sys", l + 1)
+            self._save_block("env", l + 1)
+            self._save_wave(r, l + 1, l + 1)
             r["psi"].close()
             self.history.append(("warmup", l, r["E"], r.get("sys_err", 0.0), r.get("sys_kept", 0), r["dim"]))
+            self._save_state()
             self._say(f"warm-up l={l} N_e={ntot} E={r['E']:.12f} dim={r['dim']} mv={r['matvecs']}")
             label += 1
 
@@ -359,6 +370,8 @@ class DMRG:
         r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err, guess=guess)
         self.sys_blocks[l + 1] = r["sys"]
         self.sys_trun[l + 1] = r["sys_trun"]
+        self._save_block("sys", l + 1)
+        self._save_wave(r, l + 1, N - l - 1)
         return r
 
     def _sweep_step_env(self, l, tol, kept_max, trun_err, guess=None):
@@ -367,6 +380,8 @@ class DMRG:
         r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "env", kept_max, trun_err, guess=guess)
         self.env_blocks[N - l - 1] = r["env"]
         self.env_trun[N - l - 1] = r["env_trun"]
+        self._save_block("env", N - l - 1)
+        self._save_wave(r, l + 1, N - l - 1)
         return r
 
     def _guess(self, direction, prev, sys_trun, env_trun):
@@ -390,6 +405,7 @@ class DMRG:
                 prev[0].close()
             prev = (r["psi"], r["bases"])
             self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
+            self._save_state()
             self._say(f"{tag} L->R l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
                       f"dim={r['dim']} mv={r['matvecs']}")
         if keep:
@@ -407,20 +423,102 @@ class DMRG:
                 prev[0].close()
             prev = (r["psi"], r["bases"])
             self.history.append((tag, l, r["E"], r["env_err"], r["env_kept"], r["dim"]))
+            self._save_state()
             self._say(f"{tag} R->L l={l} E={r['E']:.12f} err={r['env_err']:.3e} kept={r['env_kept']} "
                       f"dim={r['dim']} mv={r['matvecs']}")
         if prev is not None:
             prev[0].close()
 
-    def run(self):
+    # ---- checkpoint / restart (persist.py) ---------------------------------
+    def _save_block(self, side, length):
+        if not self.checkpoint:
+            return
+        from . import persist
+        blocks, truns = (self.sys_blocks, self.sys_trun) if side == "sys" else (self.env_blocks, self.env_trun)
+        d = self.checkpoint
+        persist.model_to_disk(os.path.join(d, persist.block_name(side, length)), blocks[length])
+        tr = truns[length]
+        persist.truns_to_disk(os.path.join(d, persist.trun_name(side, length)), tr, length, env=(side == "env"))
+        persist.basis_to_disk(os.path.join(d, persist.basis_name(side, length)), tr.basis, length)
+
+    def _save_wave(self, r, sys_len, env_len):
+        if not self.checkpoint:
+            return
+        from . import persist
+        b = r["bases"]
+        persist.wave_to_disk(os.path.join(self.checkpoint, persist.wave_name(1)), b.super, b.sys_bs, b.env_bs,
+                             r["psi"].download(), sys_len, env_len)
+
+    def _save_state(self):
+        """Step count and energy history after each step (state.json)."""
+        if not self.checkpoint:
+            return
+        with open(os.path.join(self.checkpoint, "state.json"), "w") as fh:
+            json.dump({"N": self.N, "Ne": self.Ne, "nstep": self.nstep, "guessed": self.guessed,
+                       "history": [[h[0], int(h[1]), float(h[2]), float(h[3]), int(h[4]), int(h[5])]
+                                   for h in self.history]}, fh)
+
+    def load_checkpoint(self, path):
+        """Blocks, truncations and step count written by a checkpointed run
+        (block_from_disk / truns_from_disk for every stored length)."""
+        from . import persist
+        from .blocks import TotalBasis
+        with open(os.path.join(path, "state.json")) as fh:
+            st = json.load(fh)
+        if (st["N"], st["Ne"]) != (self.N, self.Ne):
+            raise ValueError(f"checkpoint of N={st['N']}, N_e={st['Ne']}, not N={self.N}, N_e={self.Ne}")
+        self.nstep, self.guessed = st["nstep"], st["guessed"]
+        self.history = [tuple(h) for h in st["history"]]
+        self.sys_blocks = {1: site_model(1)}
+        self.env_blocks = {1: site_model(self.N)}
+        for side, blocks, truns in (("sys", self.sys_blocks, self.sys_trun), ("env", self.env_blocks, self.env_trun)):
+            for L in range(2, self.N):
+                f = os.path.join(path, persist.block_name(side, L))
+                if not os.path.exists(f):
+                    continue
+                blocks[L] = persist.model_from_disk(f)
+                prev = blocks.get(L - 1)
+                if prev is not None:
+                    # a block that was built from an earlier block of length
+                    # L - 1 (the pass stopped before rebuilding it) keeps no
+                    # truncation: only the measurement and the guess use them
+                    try:
+                        _, truns[L] = persist.truns_from_disk(os.path.join(path, persist.trun_name(side, L)),
+                                                              TotalBasis(prev.sectors), env=(side == "env"))
+                    except (KeyError, ValueError):
+                        truns.pop(L, None)
+
+    def run(self, checkpoint=None, restart=None):
         """Get_Ground_State (main.f90:114-209) without the final measurement
-        half-sweep; returns the last energy."""
+        half-sweep; returns the last energy. checkpoint: directory for the
+        per-length files; restart: (restart1, sys_len_start, restart2) of
+        restart.dat (main.f90:36-39): 1 skips the warm-up and starts the
+        first left-to-right half-sweep at sys_len_start (0: its usual start),
+        2 also skips that half-sweep and starts the right-to-left one at
+        sys_len_start - 1; restart2 != 0 uses the final tolerances at once
+        (main.f90:152-155). Blocks come from the checkpoint directory."""
         N = self.N
-        self.warm_up()
+        self.checkpoint = checkpoint
+        if checkpoint:
+            os.makedirs(checkpoint, exist_ok=True)
+        r1, start, r2 = restart if restart else (0, 0, 0)
+        if r1 not in (0, 1, 2):
+            raise ValueError(f"restart1 = {r1} is not 0, 1 or 2")
+        if r1 == 0:
+            self.warm_up()
+        else:
+            if not checkpoint:
+                raise ValueError("a restart reads the blocks of a checkpoint directory")
+            self.load_checkpoint(checkpoint)
         lanerr = self.lanzero
         tol, terr = lanerr * 15, self.trun_err * 15
-        self.left_to_right(N // 2 - 1, N - 3, tol, self.kept_max, terr, "half")
-        self.right_to_left(N - 3, 1, tol, self.kept_max, terr, "half")
+        if r1 != 0:
+            tol, terr = lanerr * 2, self.trun_err * 2
+        if r2 != 0:
+            tol, terr = lanerr, self.trun_err
+        if r1 <= 1:
+            self.left_to_right(start if (r1 == 1 and start) else N // 2 - 1, N - 3, tol, self.kept_max, terr, "half")
+        self.right_to_left(start - 1 if (r1 == 2 and start) else N - 3, 1, tol, self.kept_max, terr, "half")
         kept_max = self.kept_max
         for it in range(1, self.sweeps + 1):
             if it == 1:

@@ -663,3 +663,58 @@ def write_reference_files(out, dirpath, Lx, Ny, J2, kept_min, kept_max, n_electr
             lines.append(cols)
         put(name, lines)
     return paths
+
+
+def measure_from_disk(ctx, path, Lx, Ny, i0=None, js=None, kinds=("SS", "nn", "cc"), density=True,
+                      pair_bonds=(1, 3, 5), prec=capi.PREC_F64):
+    """Measurement replay (restart1 = 4 of the reference, main.f90:102-107):
+    the stored ground state eigvec1 and the stored truncations of both sides
+    (systruns<L>, envtruns<L>, persist.py layouts) are read, the blocks'
+    sector structure is rebuilt from the truncations, and the state is
+    measured as measure_state / measure_pairs do."""
+    import os
+    from types import SimpleNamespace
+    from . import persist
+    from .blocks import SuperBasis, TotalBasis
+    N = Lx * Ny
+    sys_len, env_len, subs = persist.wave_from_disk(os.path.join(path, persist.wave_name(1)))
+    Lb, Le = sys_len - 1, env_len - 1
+    if Lb + Le + 2 != N:
+        raise ValueError(f"stored wave function covers {Lb + Le + 2} sites, the lattice {N}")
+
+    def chain(side, Lend):
+        sectors = [tuple(s) for s in SITE_SECTORS]
+        truns = {}
+        for L in range(2, Lend + 1):
+            basis = TotalBasis(sectors)
+            _, tr = persist.truns_from_disk(os.path.join(path, persist.trun_name(side, L)), basis, env=(side == "env"))
+            truns[L] = tr
+            sectors = [(basis.sectors[x][0], basis.sectors[x][1], tr.U[x].shape[1]) for x in tr.order]
+        return truns, sectors
+
+    sys_truns, A_sec = chain("sys", Lb)
+    env_truns, B_sec = chain("env", Le)
+    sys_bs, env_bs = TotalBasis(A_sec), TotalBasis(B_sec)
+    (sq, eq, _) = subs[0]
+    sb = SuperBasis(sys_bs, env_bs, sq[0] + eq[0], 0)
+    psi = ctx.vec_from(persist.psi_in_super_basis(subs, sb, sys_bs, env_bs))
+    bases = SimpleNamespace(super=sb, sys_bs=sys_bs, env_bs=env_bs)
+    d_i0, d_js = reference_sites(Lx, Ny)
+    i0 = d_i0 if i0 is None else int(i0)
+    js = [j for j in (d_js if js is None else js) if j > i0]
+    try:
+        out = measure_state(ctx, psi, bases, Lb + 1, N, sys_truns, env_truns, i0, js, kinds=kinds, density=density,
+                            prec=prec)
+        out["pairs"] = {}
+        for ij1 in pair_bonds:
+            bond, partners = reference_pair_partners(Lx, Ny, Lb + 1, ij1)
+            vals = {}
+            if partners:
+                got = measure_pairs(ctx, psi, bases, Lb + 1, N, sys_truns, env_truns, bond,
+                                    [p for (_, _, p) in partners], prec=prec)
+                vals = {(sid3, t): got[tuple(sorted(p))] for (sid3, t, p) in partners}
+            out["pairs"][ij1] = {"bond": bond, "values": vals}
+    finally:
+        psi.close()
+    out["i0"], out["js"], out["l_end"] = i0, js, Lb
+    return out

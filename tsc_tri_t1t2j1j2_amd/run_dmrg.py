@@ -41,8 +41,30 @@ def main(argv=None):
     ap.add_argument("--measure", default=None, metavar="DIR",
                     help="after the sweeps, sweep left to right to 5N/6 and write the reference's measurement "
                          "files (densities, spin / hopping / density correlators, singlet pair correlators) to DIR")
+    ap.add_argument("--checkpoint", default=None, metavar="DIR",
+                    help="write every block, truncation, basis and the current ground state to DIR in the "
+                         "reference's per-length files")
+    ap.add_argument("--restart", default=None, metavar="FILE",
+                    help="restart.dat of the reference ('restart1 sys_len_start' / 'restart2'): 1 or 2 resume "
+                         "from the blocks in --checkpoint, 4 only measures the state stored there")
     a = ap.parse_args(argv)
     ctx = capi.Context(0)
+    restart = None
+    if a.restart:
+        from . import persist
+        restart = persist.read_restart(a.restart)
+        if restart[0] == 4:
+            from . import measure as meas
+            if not (a.checkpoint and a.measure):
+                ap.error("restart1 = 4 measures the state in --checkpoint and needs --measure DIR")
+            J2 = a.J1 * (a.t2 / a.t1) ** 2 if a.J2 is None else a.J2
+            t1 = time.perf_counter()
+            res = meas.measure_from_disk(ctx, a.checkpoint, a.Lx, a.Ny)
+            files = meas.write_reference_files(res, a.measure, a.Lx, a.Ny, J2, a.kept_min, a.kept_max,
+                                               int(round(sum(res["density"].values()))))
+            print(json.dumps({"measure_s": round(time.perf_counter() - t1, 2), "measure_stats": res["stats"],
+                              "measure_files": files}), flush=True)
+            return
     t0 = time.perf_counter()
 
     def log(msg):
@@ -53,7 +75,7 @@ def main(argv=None):
              kept_max=a.kept_max, kept_maxf=a.kept_maxf, sweeps=a.sweeps, lanzero=a.lanzero, trun_err=a.trun_err,
              prec=prec, refine=a.refine if prec != capi.PREC_F64 else 0, seed=a.seed, log=log,
              wave_guess=not a.no_guess, guessed_f64=a.guessed_f64, shadow_f64=a.shadow_f64)
-    E = d.run()
+    E = d.run(checkpoint=a.checkpoint, restart=restart)
     wall = time.perf_counter() - t0
     sweep_rows = [h for h in d.history if h[0].startswith("sweep")]
     last = [h for h in sweep_rows if h[0] == f"sweep{a.sweeps}"]
