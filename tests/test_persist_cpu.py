@@ -137,3 +137,22 @@ def test_measurement_replay_from_files(monkeypatch, tmp_path):
     assert rep["density"] == out["density"]
     assert rep["pairs"] == out["pairs"]
     assert os.path.exists(tmp_path / "sysbs3") and os.path.exists(tmp_path / "envtruns3")
+
+
+def test_step_logs_in_reference_formats(monkeypatch, tmp_path):
+    """Energy_All.dat (eigen.f90:192 format), Kept_State.dat and entropy1.dat
+    (den_mat.f90:342-352 formats): one line per solve / per truncation."""
+    import re
+    ctx = fake_device.install(monkeypatch)
+    d = _dmrg(ctx)
+    d.run(log_dir=str(tmp_path))
+    en = open(tmp_path / "Energy_All.dat").read().splitlines()
+    assert len(en) == len(d.history)
+    m = re.fullmatch(r"Sys_len=(.{3})  E0=  (.{18})  Iter=(.{4})  time=  (.{12})", en[-1])
+    assert m and abs(float(m.group(2)) - d.history[-1][2]) < 1e-11
+    ks = open(tmp_path / "Kept_State.dat").read().splitlines()
+    m = re.fullmatch(r"Sys_len=(.{3})  Num_State=(.{5})  Err=(.{18})", ks[-1])
+    assert m and int(m.group(2)) == d.history[-1][4]
+    assert abs(float(m.group(3)) - d.history[-1][3]) <= 1e-8 * abs(d.history[-1][3]) + 1e-300
+    ent = open(tmp_path / "entropy1.dat").read().splitlines()
+    assert len(ent) == len(ks) and all(len(line) == 8 + 20 + 27 for line in ent)

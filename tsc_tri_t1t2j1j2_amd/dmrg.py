@@ -28,6 +28,19 @@ from . import wavetrans
 from .blocks import SITE_SECTORS, TotalBlock, TotalModel, site_operators
 
 
+def fortran_e(x, w, d):
+    """Fortran Ew.d: 0.dddE+xx right-aligned in w characters."""
+    if x == 0.0:
+        m, e = 0.0, 0
+    else:
+        e = int(np.floor(np.log10(abs(x)))) + 1
+        m = x / 10.0 ** e
+        if round(abs(m), d) >= 1.0:
+            m, e = m / 10.0, e + 1
+    body = ("-" if m < 0 else "") + f"{abs(m):.{d}f}"
+    return f"{body}E{'+' if e >= 0 else '-'}{abs(e):02d}".rjust(w)
+
+
 def site_model(i):
     """Block of one site (Get_hamiltonian_site, dmrg_sub.f90:913-942)."""
     H = TotalBlock(0, 0, {(0, 0): np.zeros((1, 1), complex), (1, 1): np.zeros((1, 1), complex)})
@@ -108,6 +121,10 @@ class DMRG:
         # per-length files (persist.py), as model_to_disk / truns_to_disk /
         # basis_to_disk / wave_to_disk do at each step of the reference
         self.checkpoint = None
+        # log_dir: Energy_All.dat, Kept_State.dat and entropy1.dat lines per
+        # step in the reference's formats (eigen.f90:166-192,
+        # den_mat.f90:342-352)
+        self.log_dir = None
 
     # ---- helpers ---------------------------------------------------------
     def _say(self, msg):
@@ -149,7 +166,23 @@ class DMRG:
                 # conjugates of those of the reference's rho (den_mat.f90:89)
                 U[x] = u if side == 0 else np.conj(u)
                 order.append(x)
+        if self.log_dir:
+            n_enl = len((sb.sys if side == 0 else sb.env).sites) + 1
+            self._log_truncation(n_enl if side == 0 else self.N - n_enl, res)
         return U, order, res["trun_err"], res["num_state"]
+
+    def _log(self, name, line):
+        with open(os.path.join(self.log_dir, name), "a") as fh:
+            fh.write(line + "\n")
+
+    def _log_truncation(self, sys_len, res):
+        self._log("entropy1.dat", f"{sys_len:8d}{float(res['entropy1']):20.10f}{self.N:9d}{self.Ne:9d}{self.S2:9d}")
+        self._log("Kept_State.dat", f"Sys_len={sys_len:3d}  Num_State={int(res['num_state']):5d}  "
+                                    f"Err={fortran_e(float(res['trun_err']), 18, 8)}")
+
+    def _log_energy(self, sys_len, E, iters, seconds):
+        if self.log_dir:
+            self._log("Energy_All.dat", f"Sys_len={sys_len:3d}  E0=  {E:18.12f}  Iter={iters:4d}  time=  {seconds:12.6f}")
 
     def _initial(self, sb, guess):
         """Lanczos start: the transformed previous ground state when the
@@ -213,10 +246,12 @@ class DMRG:
         t0 = time.perf_counter()
         psi = self._initial(sb, guess)
         psi0 = self._shadow_start(psi)
+        t1 = time.perf_counter()
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
                                    max_outer=self.lanczos_budget[1], prec=self._prec_now,
                                    refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
         self.timers["lanczos"] += time.perf_counter() - t0
+        self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
         self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
@@ -277,10 +312,12 @@ class DMRG:
         t0 = time.perf_counter()
         psi = self._initial(sb, guess)
         psi0 = self._shadow_start(psi)
+        t1 = time.perf_counter()
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
                                    max_outer=self.lanczos_budget[1], prec=self._prec_now,
                                    refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
         self.timers["lanczos"] += time.perf_counter() - t0
+        self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
         self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
@@ -488,7 +525,7 @@ class DMRG:
                     except (KeyError, ValueError):
                         truns.pop(L, None)
 
-    def run(self, checkpoint=None, restart=None):
+    def run(self, checkpoint=None, restart=None, log_dir=None):
         """Get_Ground_State (main.f90:114-209) without the final measurement
         half-sweep; returns the last energy. checkpoint: directory for the
         per-length files; restart: (restart1, sys_len_start, restart2) of
@@ -496,11 +533,14 @@ class DMRG:
         first left-to-right half-sweep at sys_len_start (0: its usual start),
         2 also skips that half-sweep and starts the right-to-left one at
         sys_len_start - 1; restart2 != 0 uses the final tolerances at once
-        (main.f90:152-155). Blocks come from the checkpoint directory."""
+        (main.f90:152-155). Blocks come from the checkpoint directory.
+        log_dir: where the per-step log files go."""
         N = self.N
         self.checkpoint = checkpoint
-        if checkpoint:
-            os.makedirs(checkpoint, exist_ok=True)
+        for d in (checkpoint, log_dir):
+            if d:
+                os.makedirs(d, exist_ok=True)
+        self.log_dir = log_dir
         r1, start, r2 = restart if restart else (0, 0, 0)
         if r1 not in (0, 1, 2):
             raise ValueError(f"restart1 = {r1} is not 0, 1 or 2")

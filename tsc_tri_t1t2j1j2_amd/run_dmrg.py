@@ -47,6 +47,8 @@ def main(argv=None):
     ap.add_argument("--restart", default=None, metavar="FILE",
                     help="restart.dat of the reference ('restart1 sys_len_start' / 'restart2'): 1 or 2 resume "
                          "from the blocks in --checkpoint, 4 only measures the state stored there")
+    ap.add_argument("--log-dir", default=None, metavar="DIR",
+                    help="append Energy_All.dat, Kept_State.dat and entropy1.dat lines per step to DIR")
     a = ap.parse_args(argv)
     ctx = capi.Context(0)
     restart = None
@@ -75,7 +77,7 @@ def main(argv=None):
              kept_max=a.kept_max, kept_maxf=a.kept_maxf, sweeps=a.sweeps, lanzero=a.lanzero, trun_err=a.trun_err,
              prec=prec, refine=a.refine if prec != capi.PREC_F64 else 0, seed=a.seed, log=log,
              wave_guess=not a.no_guess, guessed_f64=a.guessed_f64, shadow_f64=a.shadow_f64)
-    E = d.run(checkpoint=a.checkpoint, restart=restart)
+    E = d.run(checkpoint=a.checkpoint, restart=restart, log_dir=a.log_dir)
     wall = time.perf_counter() - t0
     sweep_rows = [h for h in d.history if h[0].startswith("sweep")]
     last = [h for h in sweep_rows if h[0] == f"sweep{a.sweeps}"]
