@@ -44,7 +44,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="yc6-m6000", choices=["yc6-m6000", "synthetic400", "synthetic-small"])
+    ap.add_argument("--workload", default="yc6-m6000",
+                    choices=["yc6-m6000", "yc6-m12000", "synthetic400", "synthetic-small"])
     ap.add_argument("--no-lanczos", action="store_true", help="H.psi only (no Lanczos, truncation or DMRG check)")
     ap.add_argument("--lanczos-iters", type=int, default=40)
     ap.add_argument("--refine", type=int, default=4, help="FP64 Lanczos steps after the mixed ones")
@@ -106,6 +107,13 @@ def build_workload(name):
                 "model": "t1=1 t2=0.3 J1=0.4 J2=0.036", "psi_multiplet_dim": wl.psi_len,
                 "sectors": wl.sb.super.num, "plan_build_s": round(wl.plan_seconds, 2)}
         return wl, desc
+    if name == "yc6-m12000":
+        wl = PhysicsWorkload(24, 6, 12000, t2=0.2)
+        desc = {"workload": "YC6 6x24 (144 sites, 132 electrons, S=0) centre superblock at M=12000 SU(2) "
+                            "multiplets: real t1-t2-J1-J2 term list and sector structure, seeded random blocks",
+                "model": "t1=1 t2=0.2 J1=0.4 J2=0.016", "psi_multiplet_dim": wl.psi_len,
+                "sectors": wl.sb.super.num, "plan_build_s": round(wl.plan_seconds, 2)}
+        return wl, desc
     if name == "synthetic400":
         sb = SyntheticSuperblock(nsec=400, nterms=24, dmin=8, dmax=1536)
         desc = {"workload": "synthetic superblock H.psi: 400 SU(2) sectors, d_L,d_R log-uniform [8,1536], "
@@ -162,7 +170,7 @@ def time_gpu(ctx, plan, vin, vout, prec, steps, warmup):
     return wall, float(np.mean(ev1)), float(np.mean(ev2))
 
 
-def lanczos_and_step(ctx, plan, wl, vin, prec, args):
+def lanczos_and_step(ctx, plan, wl, vin, prec, args, hbm_mark=None):
     """Device time of one DMRG step's hot path on the centre superblock in
     FP64 and in mixed precision: Lanczos at a fixed Krylov budget (FP64: `it`
     H.psi; mixed: `it` mixed H.psi then `refine` FP64 H.psi restarted from the
@@ -184,6 +192,8 @@ def lanczos_and_step(ctx, plan, wl, vin, prec, args):
     t0 = time.perf_counter()
     em, (mvm, _) = plan.lanczos(psim, tol=1e-13, max_inner=it, max_outer=0, prec=prec, refine_iters=refine)
     tm = time.perf_counter() - t0
+    if hbm_mark is not None:
+        hbm_mark()
     print(f"# lanczos f64 {t64:.2f}s ({mv64} H.psi) E={e64:.12f}; mixed {tm:.2f}s ({mvm} H.psi) E={em:.12f}",
           file=sys.stderr, flush=True)
     sbk = wl.sb
@@ -191,10 +201,10 @@ def lanczos_and_step(ctx, plan, wl, vin, prec, args):
     ed = [dy for (_, _, _, dy) in sbk.super.sub]
     twos = [sbk.sys_bs.sectors[x][1] for (x, _, _, _) in sbk.super.sub]
     # the first call also creates the solver handles and streams: time the second
-    capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, 6000, 1e-15)
+    capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, wl.M, 1e-15)
     ctx.sync()
     t0 = time.perf_counter()
-    tr = capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, 6000, 1e-15)
+    tr = capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, wl.M, 1e-15)
     ttr = time.perf_counter() - t0
     s64, smx = t64 + ttr, tm + ttr
     return {"lanczos_budget": f"FP64 {it} H.psi; mixed {it} + {refine} FP64 refinement H.psi",
@@ -364,6 +374,14 @@ def main():
     sb, desc = build_workload(args.workload)
     t_setup = time.perf_counter()
     ctx = capi.Context(0)
+    free0, total = ctx.mem_info()
+    hbm = {"peak": 0}
+
+    def hbm_mark():
+        # device memory in use beyond what was in use when the context was made
+        free, _ = ctx.mem_info()
+        hbm["peak"] = max(hbm["peak"], free0 - free)
+
     plan = sb.make_plan(ctx) if isinstance(sb, PhysicsWorkload) else sb.make_plan(ctx, None)
     vin = ctx.vec(sb.psi_len)
     vin.fill_gaussian(getattr(sb, "seed", 20261203), 0xC0FFEE, 1.0)
@@ -383,12 +401,15 @@ def main():
         f1, f2 = stage_flops(sb)
 
     # FP64 reference application and timing
+    hbm_mark()
     w64, k1_64, k2_64 = time_gpu(ctx, plan, vin, vref, capi.PREC_F64, args.steps, args.warmup)
+    hbm_mark()
     print(f"# f64: {w64 * 1e3:.2f} ms/step, {F / w64 / 1e12:.2f} TFLOP/s (stage 1 {k1_64:.2f} ms, stage 2 "
           f"{k2_64:.2f} ms)", file=sys.stderr, flush=True)
     ref = vref.download()
     if prec != capi.PREC_F64:
         wall, k1, k2 = time_gpu(ctx, plan, vin, vout, prec, args.steps, args.warmup)
+        hbm_mark()
         out = vout.download()
         rel = float(np.linalg.norm(out - ref) / np.linalg.norm(ref))
     else:
@@ -410,7 +431,7 @@ def main():
 
     extra = {}
     if isinstance(sb, PhysicsWorkload) and not args.no_lanczos:
-        extra = lanczos_and_step(ctx, plan, sb, vin, prec, args)
+        extra = lanczos_and_step(ctx, plan, sb, vin, prec, args, hbm_mark)
         if not args.no_full_step:
             plan.close()
             extra.update(full_step(ctx, sb, prec, args))
@@ -458,6 +479,8 @@ def main():
         "cpu_baseline": cpu,
     }
     line["config"].update(extra)
+    line["config"]["hbm_peak_gib"] = round(hbm["peak"] / 2 ** 30, 2)
+    line["config"]["hbm_total_gib"] = round(total / 2 ** 30, 1)
     print(json.dumps(line), flush=True)
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * (1 << 20) // 16

@@ -108,6 +108,13 @@ int tsc_ctx_sync(tsc_ctx *ctx);
  */
 int tsc_ctx_mfma_layout(const tsc_ctx *ctx, int32_t *modes3);
 
+/*
+ * Free and total device memory (bytes) of the context's GPU, as the HIP
+ * runtime reports them (every process's allocations count). Diagnostic for
+ * HBM footprints; has no counterpart in the reference.
+ */
+int tsc_ctx_mem_info(const tsc_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* ---- resident vectors (psi, Lanczos vectors) ---------------------------- */
 int tsc_vec_create(tsc_ctx *ctx, int64_t n, tsc_vec **out);
 int tsc_vec_destroy(tsc_vec *v);

@@ -35,6 +35,7 @@ OUTBLOCK_DTYPE = np.dtype({
 # every symbol declared in include/tsc_hotpath.h
 EXPORTS = [
     "tsc_last_error", "tsc_version", "tsc_ctx_create", "tsc_ctx_destroy", "tsc_ctx_sync", "tsc_ctx_mfma_layout",
+    "tsc_ctx_mem_info",
     "tsc_vec_create", "tsc_vec_destroy", "tsc_vec_upload", "tsc_vec_download", "tsc_vec_dot_re",
     "tsc_vec_norm", "tsc_vec_dptr", "tsc_vec_fill_gaussian", "tsc_vec_axpby", "tsc_vec_copy",
     "tsc_vec_copy_range", "tsc_vec_upload_range", "tsc_vec_download_range",
@@ -77,6 +78,7 @@ def lib():
         "tsc_ctx_destroy": (c_int, [c_void_p]),
         "tsc_ctx_sync": (c_int, [c_void_p]),
         "tsc_ctx_mfma_layout": (c_int, [c_void_p, P(c_int32)]),
+        "tsc_ctx_mem_info": (c_int, [c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64)]),
         "tsc_vec_create": (c_int, [c_void_p, c_int64, P(c_void_p)]),
         "tsc_vec_destroy": (c_int, [c_void_p]),
         "tsc_vec_upload": (c_int, [c_void_p, c_void_p]),
@@ -164,6 +166,12 @@ class Context:
 
     def sync(self):
         check(lib().tsc_ctx_sync(self.h))
+
+    def mem_info(self):
+        """(free, total) device memory in bytes."""
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().tsc_ctx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
 
     def vec(self, n):
         return Vec(self, n)

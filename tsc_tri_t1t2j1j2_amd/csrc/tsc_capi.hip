@@ -134,6 +134,16 @@ extern "C" int tsc_ctx_mfma_layout(const tsc_ctx *c, int32_t *modes) {
   return TSC_OK;
 }
 
+extern "C" int tsc_ctx_mem_info(const tsc_ctx *c, uint64_t *free_bytes, uint64_t *total_bytes) {
+  TSC_REQUIRE(c && free_bytes && total_bytes, "tsc_ctx_mem_info arguments");
+  TSC_HIP_CHECK(hipSetDevice(c->device));
+  size_t f = 0, t = 0;
+  TSC_HIP_CHECK(hipMemGetInfo(&f, &t));
+  *free_bytes = (uint64_t)f;
+  *total_bytes = (uint64_t)t;
+  return TSC_OK;
+}
+
 extern "C" int tsc_ctx_create(int device, tsc_ctx **out) {
   TSC_REQUIRE(out, "tsc_ctx_create: null out");
   int ndev = 0;

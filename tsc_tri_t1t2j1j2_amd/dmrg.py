@@ -109,6 +109,7 @@ class DMRG:
         self.log = log
         self.nstep = 0
         self.hpsi_flops = 0.0
+        self.hpsi_calls = 0
         self.timers = {"plan": 0.0, "guess": 0.0, "lanczos": 0.0, "truncate": 0.0, "renorm": 0.0}
         self.guessed = 0
         # U^dagger O U contractions: algorithmic flops, device ms; rho and
@@ -254,6 +255,7 @@ class DMRG:
         self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
         self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
+        self.hpsi_calls += nmv
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
         if grow in ("sys", "both"):
             U, order, err, ns = self._truncate(sb, psi, 0, self.kept_min, kept_max, trun_err, identity_sys)
@@ -320,6 +322,7 @@ class DMRG:
         self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
         self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
+        self.hpsi_calls += nmv
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
         for side, idx, ident in (("sys", 0, identity_sys), ("env", 1, identity_env)):
             if side not in spec:

@@ -87,7 +87,8 @@ def main(argv=None):
         "E_min_last_sweep": min(h[2] for h in last) if last else E,
         "max_trun_err_last_sweep": max(h[3] for h in last) if last else None,
         "wall_s": round(wall, 2), "timers_s": {k: round(v, 2) for k, v in d.timers.items()},
-        "hpsi_flops_total": d.hpsi_flops, "steps": len(d.history), "guessed_steps": d.guessed,
+        "hpsi_flops_total": d.hpsi_flops, "hpsi_calls": d.hpsi_calls, "steps": len(d.history),
+        "guessed_steps": d.guessed, "lanzero": a.lanzero,
     }
     if d.shadow:
         de = [abs(em - e64) / d.N for (_, em, e64) in d.shadow]
