@@ -48,7 +48,7 @@ def parse():
                     choices=["yc6-m6000", "yc6-m12000", "synthetic400", "synthetic-small"])
     ap.add_argument("--no-lanczos", action="store_true", help="H.psi only (no Lanczos, truncation or DMRG check)")
     ap.add_argument("--lanczos-iters", type=int, default=40)
-    ap.add_argument("--refine", type=int, default=4, help="FP64 Lanczos steps after the mixed ones")
+    ap.add_argument("--refine", type=int, default=2, help="FP64 Lanczos steps after the mixed ones")
     ap.add_argument("--no-dmrg", action="store_true", help="skip the FP64-vs-mixed DMRG energy check")
     ap.add_argument("--no-full-step", action="store_true", help="skip the resident full-step timing")
     ap.add_argument("--dmrg-sweeps", type=int, default=6)

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "tsc_internal.h"
 
@@ -123,7 +124,7 @@ __global__ void eig_unpad_kernel(double2 *__restrict__ rho, const double2 *__res
 }
 
 constexpr int kJacobiMax = 64;  // largest sector solved in a padded Jacobi batch
-constexpr int kEigStreams = 8;  // side streams for concurrent eigensolves
+constexpr int kEigStreams = 16;  // side streams (and submitting threads) for concurrent eigensolves
 
 // Side streams (default priority) with one rocBLAS handle each, per device
 // and host thread, kept for the life of the process.
@@ -162,7 +163,8 @@ int eig_lanes(tsc_ctx *ctx, EigLanes **out) {
 // ascending eigensolve of every rho_s (ZHEEV 'V','U' role, den_mat.f90:229-253):
 // eigenvectors replace rho_s in place, eigenvalues go to d_w. Buckets run
 // concurrently on the side streams (the solver is launch-bound at these
-// sizes), largest first onto the least-loaded stream.
+// sizes), largest first onto the least-loaded stream, one submitting host
+// thread per stream.
 int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, const std::vector<int64_t> &rho_off,
                 double *d_w, const std::vector<int64_t> &w_off) {
   EigLanes *L = nullptr;
@@ -218,10 +220,16 @@ int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, 
   auto work = [&](const Bucket &bk) { return (double)bk.cnt * bk.P * bk.P * bk.P; };
   std::stable_sort(border.begin(), border.end(), [&](size_t x, size_t y) { return work(buckets[x]) > work(buckets[y]); });
   std::vector<double> load(kEigStreams, 0.0);
+  std::vector<std::vector<size_t>> lane_buckets(kEigStreams);
   for (size_t bi : border) {
-    const Bucket &bk = buckets[bi];
     const int lane = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    load[lane] += work(bk) + 1e6;  // fixed cost per call
+    load[lane] += work(buckets[bi]) + 1e6;  // fixed cost per call
+    lane_buckets[lane].push_back(bi);
+  }
+  // one bucket on its lane: pad, solve, unpad; returns 0, 1 (launch) or 2
+  // (solver status), and never touches the shared error string
+  auto run_bucket = [&](size_t bi, int lane) -> int {
+    const Bucket &bk = buckets[bi];
     hipStream_t ls = L->stream[lane];
     rocblas_handle h = L->handle[lane];
     const int P = bk.P, cnt = (int)bk.cnt;
@@ -233,7 +241,7 @@ int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, 
     dim3 grid((unsigned)((pp + 255) / 256), (unsigned)cnt);
     // slot offsets address the shared buffer; pass its base
     hipLaunchKernelGGL(eig_pad_kernel, grid, dim3(256), 0, ls, (const double2 *)d_rho, (double2 *)buf.p, dslot, P);
-    TSC_HIP_CHECK(hipGetLastError());
+    if (hipGetLastError() != hipSuccess) return 1;
     rocblas_status rs;
     if (P > kJacobiMax) {
       // a large sector alone: no padding (P == d), divide and conquer
@@ -247,14 +255,35 @@ int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, 
                                             (double *)resid.p + bk.slot0, 100, (rocblas_int *)sweeps.p + bk.slot0,
                                             bw, P, binfo, cnt);
     }
-    if (rs != rocblas_status_success) {
-      set_error("rocSOLVER Hermitian eigensolver failed (n = " + std::to_string(P) + ", batch " +
-                std::to_string(cnt) + ")");
+    if (rs != rocblas_status_success) return 2;
+    hipLaunchKernelGGL(eig_unpad_kernel, grid, dim3(256), 0, ls, d_rho, (const double2 *)buf.p, bw, d_w, dslot, P);
+    return hipGetLastError() != hipSuccess ? 1 : 0;
+  };
+  // Each lane is fed by its own host thread: the divide-and-conquer solver
+  // waits on the host inside a call, so a single submitting thread would
+  // leave the other lanes idle meanwhile.
+  std::vector<int> lane_rc(kEigStreams, 0);
+  {
+    std::vector<std::thread> pool;
+    const int dev = ctx->device;
+    for (int lane = 0; lane < kEigStreams; ++lane) {
+      if (lane_buckets[lane].empty()) continue;
+      pool.emplace_back([&, lane, dev]() {
+        if (hipSetDevice(dev) != hipSuccess) {
+          lane_rc[lane] = 1;
+          return;
+        }
+        for (size_t bi : lane_buckets[lane])
+          if ((lane_rc[lane] = run_bucket(bi, lane)) != 0) return;
+      });
+    }
+    for (auto &t : pool) t.join();
+  }
+  for (int lane = 0; lane < kEigStreams; ++lane)
+    if (lane_rc[lane]) {
+      set_error(lane_rc[lane] == 2 ? "rocSOLVER Hermitian eigensolver failed" : "eigensolver launch failed");
       return TSC_EHIP;
     }
-    hipLaunchKernelGGL(eig_unpad_kernel, grid, dim3(256), 0, ls, d_rho, (const double2 *)buf.p, bw, d_w, dslot, P);
-    TSC_HIP_CHECK(hipGetLastError());
-  }
   for (int i = 0; i < kEigStreams; ++i) {
     TSC_HIP_CHECK(hipEventRecord(L->done[i], L->stream[i]));
     TSC_HIP_CHECK(hipStreamWaitEvent(st, L->done[i], 0));

@@ -30,7 +30,7 @@ def main(argv=None):
     ap.add_argument("--lanzero", type=float, default=1e-10)
     ap.add_argument("--trun-err", type=float, default=1e-15)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "bf16x3"])
-    ap.add_argument("--refine", type=int, default=4)
+    ap.add_argument("--refine", type=int, default=2)
     ap.add_argument("--no-guess", action="store_true", help="start every Lanczos from a random vector")
     ap.add_argument("--guessed-f64", action="store_true",
                     help="with a mixed precision, solve the steps that start from the guess in FP64")
