@@ -308,6 +308,7 @@ def dmrg_precision(ctx, sweeps, refine):
     from tsc_tri_t1t2j1j2_amd.dmrg import DMRG
     out = {}
     shadow = []
+    by_phase = {}
     for name, prec, rf in (("f64", capi.PREC_F64, 0), ("bf16x3", capi.PREC_BF16X3, refine)):
         t0 = time.perf_counter()
         d = DMRG(ctx, 6, 4, kept_min=110, kept_max=256, sweeps=sweeps, lanzero=1e-10, prec=prec, refine=rf,
@@ -315,6 +316,7 @@ def dmrg_precision(ctx, sweeps, refine):
         E = d.run()
         out[name] = (E, time.perf_counter() - t0, d.N)
         shadow = shadow or d.shadow
+        by_phase = by_phase or d.shadow_by_phase()
         print(f"# dmrg YC4 6x4 M=256 {name}: E={E:.12f} in {out[name][1]:.1f}s", file=sys.stderr, flush=True)
     N = out["f64"][2]
     per_step = [abs(em - e64) / N for (_, em, e64) in shadow]
@@ -324,6 +326,7 @@ def dmrg_precision(ctx, sweeps, refine):
             "dmrg_same_path_steps": len(per_step),
             "dmrg_same_path_max_dE_per_site": max(per_step) if per_step else None,
             "dmrg_same_path_last_dE_per_site": per_step[-1] if per_step else None,
+            "dmrg_same_path_max_dE_per_site_by_phase": {k: float(f"{v:.3e}") for k, v in by_phase.items()},
             "dmrg_wall_s_f64": round(out["f64"][1], 2),
             "dmrg_wall_s_mixed_with_fp64_shadow": round(out["bf16x3"][1], 2)}
 

@@ -35,3 +35,20 @@ def test_resident_driver_matches_host_driver(monkeypatch):
         runs.append([h[2] for h in d.history])
     assert len(runs[0]) == len(runs[1])
     assert np.allclose(runs[0], runs[1], atol=1e-9, rtol=0)
+
+
+def test_shadow_error_by_phase(monkeypatch):
+    """With FP64 shadow solves the per-phase maximum of |E_mixed - E_FP64| /
+    site is reported for the warm-up, both half-sweeps and every sweep."""
+    import fake_device
+    from tsc_tri_t1t2j1j2_amd import capi
+    from tsc_tri_t1t2j1j2_amd.dmrg import DMRG
+    ctx = fake_device.install(monkeypatch)
+    d = DMRG(ctx, 3, 3, Ne=8, kept_min=4, kept_max=12, sweeps=2, lanzero=1e-10, trun_err=1e-14,
+             native_planner=False, prec=capi.PREC_BF16X3, shadow_f64=True)
+    d.run()
+    ph = d.shadow_by_phase()
+    assert set(ph) == {"warmup", "half", "sweep1", "sweep2"}
+    assert len(d.shadow) == len(d.history)
+    # the fake device solves every precision in FP64: both solves agree
+    assert max(ph.values()) < 1e-12

@@ -216,6 +216,16 @@ class DMRG:
         psi0.close()
         self.shadow.append((self.nstep, E, E64))
 
+    def shadow_by_phase(self):
+        """Largest same-superblock |E_mixed - E_FP64| / site per phase of the
+        run (warm-up, the half-sweeps, each sweep), from the FP64 shadow
+        solves: the Ritz-energy error of the mixed precision, per sweep."""
+        out = {}
+        for (k, em, e64) in self.shadow:
+            tag = self.history[k][0] if k < len(self.history) else "final"
+            out[tag] = max(out.get(tag, 0.0), abs(em - e64) / self.N)
+        return out
+
     def _dev(self, model):
         """The block with its operators on the device (uploaded once)."""
         if getattr(model, "arena", None) is not None:

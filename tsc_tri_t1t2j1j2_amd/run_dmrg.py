@@ -94,6 +94,7 @@ def main(argv=None):
         de = [abs(em - e64) / d.N for (_, em, e64) in d.shadow]
         out["same_path_max_dE_per_site"] = max(de)
         out["same_path_last_dE_per_site"] = de[-1]
+        out["same_path_max_dE_per_site_by_phase"] = d.shadow_by_phase()
     if a.measure:
         from . import measure as meas
         t1 = time.perf_counter()
