@@ -368,3 +368,37 @@ def test_update_trun_matches_oracle(ctx, prec):
         assert g.shape == w.shape
         if w.size:
             assert _rel(g, w) < (1e-13 if prec == capi.PREC_F64 else 1e-5)
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_F64, capi.PREC_BF16X3])
+def test_work_unit_targets_and_autotuner(ctx, prec):
+    """Every work-unit cut the autotuner may choose (long contribution lists
+    split into units whose partial tiles the reduce kernel sums) gives the
+    oracle's H.psi, graph replays after a re-cut use the new units, and a
+    plan of a tuned shape takes the cached choice."""
+    sb = SyntheticSuperblock(nsec=30, nterms=10, dmin=8, dmax=300, seed=31)
+    psi = sb.psi_host()
+    ref = osynth.hpsi(sb, psi)
+    plan = sb.make_plan(ctx, sb.ops_host())
+    vin = ctx.vec_from(psi)
+    vout = ctx.vec(sb.psi_len)
+    plan.apply_graph(vin, vout)             # a graph of the first units
+    ctx.sync()
+    for target in (512.0, 2048.0, 4096.0, 8192.0, 32768.0):
+        plan.autotune(vin, vout, prec, target=target)
+        vout.fill_gaussian(3, 1, 1.0)
+        plan.apply(vin, vout, prec)
+        ctx.sync()
+        eager = vout.download()
+        assert _rel(eager, ref) < TOL[prec], target
+        vout.fill_gaussian(3, 2, 1.0)
+        plan.apply_graph(vin, vout, prec)
+        ctx.sync()
+        assert _rel(vout.download(), eager) < 1e-15, target
+    chosen = plan.autotune(vin, vout, prec, force=True)
+    assert chosen in (2048.0, 4096.0, 8192.0)
+    plan2 = sb.make_plan(ctx, sb.ops_host())
+    assert plan2.autotune(vin, vout, prec) == chosen
+    got = plan2.apply_host(psi, prec)
+    assert _rel(got, ref) < TOL[prec]
+    assert capi.Plan.autotune(plan, vin, vout, capi.PREC_F32) == 0.0

@@ -4,7 +4,9 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 
 #include "tsc_internal.h"
 
@@ -648,6 +650,85 @@ extern "C" int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int 
   TSC_HIP_CHECK(hipGraphLaunch(p->gexec[prec], s));
   TSC_HIP_CHECK(hipEventRecord(p->ev[2], s));
   p->timed = false;
+  return TSC_OK;
+}
+
+namespace {
+// tuned work-unit targets, shared by plans of the same coarse shape:
+// (precision, 2 log2 flops, 2 log2 stage-2 output blocks) -> make_tiles target
+std::map<std::tuple<int, int, int>, double> g_tune_cache;
+std::mutex g_tune_mu;
+
+void drop_graph(tsc_plan *p, int prec) {
+  if (p->gexec[prec]) hipGraphExecDestroy(p->gexec[prec]);
+  p->gexec[prec] = nullptr;
+  p->g_in[prec] = p->g_out[prec] = nullptr;
+}
+}  // namespace
+
+extern "C" int tsc_plan_autotune(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int prec, int flags,
+                                 double *chosen) {
+  TSC_REQUIRE(p && in && out && chosen, "tsc_plan_autotune arguments");
+  TSC_REQUIRE(prec >= TSC_PREC_F64 && prec <= TSC_PREC_BF16X3, "unknown precision");
+  TSC_REQUIRE(in->n >= p->in_elems && out->n >= p->out_elems && in->d != out->d, "tsc_plan_autotune vectors");
+  int rc = ensure_precision(p, prec);
+  if (rc) return rc;
+  if (prec == TSC_PREC_F32) {  // the FP32 path keeps its fixed units
+    *chosen = 0.0;
+    return TSC_OK;
+  }
+  auto retile = [&](double d) { return prec == TSC_PREC_F64 ? gd_retile(p, d) : pk_retile(p, d); };
+  if (flags & 2) {
+    // a given target, no timing
+    const double d = *chosen;
+    TSC_REQUIRE(d >= 64.0 && d <= 1e7, "tsc_plan_autotune: work-unit target outside [64, 1e7]");
+    rc = retile(d);
+    drop_graph(p, prec);
+    return rc;
+  }
+  const auto key = std::make_tuple(prec, (int)std::lround(2.0 * std::log2(std::max(p->flops, 1.0))),
+                                   (int)std::lround(2.0 * std::log2(std::max<double>((double)p->h_o[1].size(), 1.0))));
+  if (!(flags & 1)) {
+    double v = 0.0;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) v = it->second;
+    }
+    if (v > 0.0) {
+      rc = retile(v);
+      drop_graph(p, prec);
+      *chosen = v;
+      return rc;
+    }
+  }
+  // time each candidate: one H.psi to settle, then three on the plan's events
+  static const double cand[] = {2048.0, 4096.0, 8192.0};
+  hipStream_t s = p->ctx->stream;
+  double best = cand[0], best_ms = 1e300;
+  for (double d : cand) {
+    if ((rc = retile(d))) return rc;
+    if ((rc = enqueue_hpsi(p, in->d, out->d, prec, false))) return rc;
+    TSC_HIP_CHECK(hipEventRecord(p->ev[0], s));
+    for (int r = 0; r < 3; ++r)
+      if ((rc = enqueue_hpsi(p, in->d, out->d, prec, false))) return rc;
+    TSC_HIP_CHECK(hipEventRecord(p->ev[2], s));
+    TSC_HIP_CHECK(hipEventSynchronize(p->ev[2]));
+    float ms = 0.f;
+    TSC_HIP_CHECK(hipEventElapsedTime(&ms, p->ev[0], p->ev[2]));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = d;
+    }
+  }
+  if ((rc = retile(best))) return rc;
+  drop_graph(p, prec);
+  p->timed = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  *chosen = best;
   return TSC_OK;
 }
 

@@ -374,7 +374,7 @@ int gd_build_impl(tsc_plan *p, GdState *S) {
         return x.coef_im < y.coef_im;
       });
     }
-    Units un = make_tiles(cs.data(), O.data(), (int32_t)O.size(), GM, GN, 2048.0);
+    Units un = make_tiles(cs.data(), O.data(), (int32_t)O.size(), GM, GN, S->desired);
     GdStage &G = S->st[s];
     G.nt = (int)un.tiles.size();
     G.nr = (int)un.reds.size();
@@ -383,6 +383,7 @@ int gd_build_impl(tsc_plan *p, GdState *S) {
     if ((rc = gd_upload(&G.d_c, cs, st))) return rc;
     if ((rc = gd_upload(&G.d_t, un.tiles, st))) return rc;
     if ((rc = gd_upload(&G.d_r, un.reds, st))) return rc;
+    G.h_c.swap(cs);
   }
   if (slots > 0) {
     hipError_t e = hipMalloc((void **)&S->d_part, sizeof(double2) * (size_t)GM * GN * (size_t)slots);
@@ -390,6 +391,7 @@ int gd_build_impl(tsc_plan *p, GdState *S) {
       set_error(std::string("hipMalloc(FP64 partial tiles) failed: ") + hipGetErrorString(e));
       return TSC_ENOMEM;
     }
+    S->part_slots = slots;
   }
   TSC_HIP_CHECK(hipStreamSynchronize(st));
   return TSC_OK;
@@ -417,6 +419,47 @@ int gd_build(tsc_plan *p) {
     return rc;
   }
   p->gd = S;
+  return TSC_OK;
+}
+
+int gd_retile(tsc_plan *p, double desired) {
+  GdState *S = p->gd;
+  if (!S) {
+    set_error("FP64 path re-cut before it was built");
+    return TSC_ESTATE;
+  }
+  hipStream_t st = p->ctx->stream;
+  // the current units may still be in use by queued launches
+  TSC_HIP_CHECK(hipStreamSynchronize(st));
+  int64_t slots = 0;
+  for (int s = 0; s < 2; ++s) {
+    GdStage &G = S->st[s];
+    const std::vector<tsc_outblock> &O = p->h_o[s];
+    Units un = make_tiles(G.h_c.data(), O.data(), (int32_t)O.size(), GM, GN, desired);
+    hipFree(G.d_t);
+    hipFree(G.d_r);
+    G.d_t = nullptr;
+    G.d_r = nullptr;
+    G.nt = (int)un.tiles.size();
+    G.nr = (int)un.reds.size();
+    slots = std::max(slots, un.nslot);
+    int rc;
+    if ((rc = gd_upload(&G.d_t, un.tiles, st))) return rc;
+    if ((rc = gd_upload(&G.d_r, un.reds, st))) return rc;
+  }
+  if (slots > S->part_slots) {
+    hipFree(S->d_part);
+    S->d_part = nullptr;
+    S->part_slots = 0;
+    hipError_t e = hipMalloc((void **)&S->d_part, sizeof(double2) * (size_t)GM * GN * (size_t)slots);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc(FP64 partial tiles) failed: ") + hipGetErrorString(e));
+      return TSC_ENOMEM;
+    }
+    S->part_slots = slots;
+  }
+  S->desired = desired;
+  TSC_HIP_CHECK(hipStreamSynchronize(st));
   return TSC_OK;
 }
 

@@ -115,6 +115,7 @@ struct PkJob {
 };
 
 struct PkStage {
+  std::vector<tsc_contrib> h_cp;  // host copy of d_cp (work estimates when the units are re-cut)
   PkContrib *d_pc = nullptr;   // packed-path contribution records (permuted order)
   tsc_contrib *d_cp = nullptr; // the same contributions, original records (elementwise terms)
   PkOut *d_po = nullptr;       // stage-1 blocks: packed destinations
@@ -133,6 +134,8 @@ struct PkState {
   int n_ops_jobs = 0, n_in_jobs = 0;
   PkStage st[2];
   void *d_part = nullptr;  // float2 partial tiles
+  int64_t part_slots = 0;  // capacity of d_part in 128x128 tiles
+  double desired = 2048.0; // work-unit target of make_tiles for both stages
 };
 
 void pk_free(PkState *s);
@@ -141,6 +144,7 @@ void pk_free(PkState *s);
 // complex128 buffers.
 constexpr int kGdTile = 128;
 struct GdStage {
+  std::vector<tsc_contrib> h_c;  // host copy of d_c (work estimates when the units are re-cut)
   tsc_contrib *d_c = nullptr;  // contributions, each block's GEMM terms first
   Tile *d_t = nullptr;
   Reduce *d_r = nullptr;
@@ -149,6 +153,8 @@ struct GdStage {
 struct GdState {
   GdStage st[2];
   double2 *d_part = nullptr;  // partial tiles of split units
+  int64_t part_slots = 0;     // capacity of d_part in units
+  double desired = 2048.0;    // work-unit target of make_tiles for both stages
 };
 void gd_free(GdState *s);
 
@@ -231,6 +237,9 @@ int pk_repack_ops(tsc_plan *p);    // after the arena changed
 int pk_enqueue(tsc_plan *p, const double2 *in, double2 *out, bool events);
 int gd_build(tsc_plan *p);  // FP64 units and contribution order, on first use
 int gd_enqueue(tsc_plan *p, const double2 *in, double2 *out, bool events);
+// cut the work units of a built path again for another make_tiles target
+int pk_retile(tsc_plan *p, double desired);
+int gd_retile(tsc_plan *p, double desired);
 // work units of one stage (shared by all paths)
 struct Units {
   std::vector<Tile> tiles;

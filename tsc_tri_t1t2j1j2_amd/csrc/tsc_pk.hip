@@ -761,7 +761,7 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
     // 8 waves for blocks 129..256 wide read each A step once instead of twice,
     // 19% less HBM traffic, but ran slower at YC6 M=6000: one workgroup per
     // CU and a second launch with its own tail.)
-    Units un = make_tiles(cps[s].data(), p->h_o[s].data(), (int32_t)p->h_o[s].size(), WT, WT, 2048.0);
+    Units un = make_tiles(cps[s].data(), p->h_o[s].data(), (int32_t)p->h_o[s].size(), WT, WT, S->desired);
     G.nt = (int)un.tiles.size();
     G.nr = (int)un.reds.size();
     G.nslot = un.nslot;
@@ -770,9 +770,11 @@ int pk_build_impl(tsc_plan *p, PkState *S) {
     if ((rc = upload_vec(&G.d_cp, cps[s], st))) return rc;
     if ((rc = upload_vec(&G.d_t, un.tiles, st))) return rc;
     if ((rc = upload_vec(&G.d_r, un.reds, st))) return rc;
+    G.h_cp.swap(cps[s]);
   }
   if ((rc = upload_vec(&S->st[0].d_po, pouts, st))) return rc;
   if ((rc = alloc_bytes(&S->d_part, slots * (int64_t)WT * WT * (int64_t)sizeof(float2), "partial tiles"))) return rc;
+  S->part_slots = slots;
   if ((rc = launch_pack(p->d_ops64, S->d_ops, S->d_ops_panels, S->d_ops_jobs, S->n_ops_jobs, st))) return rc;
   TSC_HIP_CHECK(hipStreamSynchronize(st));
   return TSC_OK;
@@ -802,6 +804,44 @@ int pk_repack_ops(tsc_plan *p) {
   PkState *S = p->pk;
   if (!S || !S->unsupported.empty() || !S->d_ops) return TSC_OK;
   return launch_pack(p->d_ops64, S->d_ops, S->d_ops_panels, S->d_ops_jobs, S->n_ops_jobs, p->ctx->stream);
+}
+
+int pk_retile(tsc_plan *p, double desired) {
+  PkState *S = p->pk;
+  if (!S || !S->unsupported.empty()) {
+    set_error("BF16x3 path re-cut before it was built");
+    return TSC_ESTATE;
+  }
+  hipStream_t st = p->ctx->stream;
+  // the current units may still be in use by queued launches
+  TSC_HIP_CHECK(hipStreamSynchronize(st));
+  int64_t slots = 0;
+  int rc;
+  for (int s = 0; s < 2; ++s) {
+    PkStage &G = S->st[s];
+    Units un = make_tiles(G.h_cp.data(), p->h_o[s].data(), (int32_t)p->h_o[s].size(), WT, WT, desired);
+    hipFree(G.d_t);
+    hipFree(G.d_r);
+    G.d_t = nullptr;
+    G.d_r = nullptr;
+    G.nt = (int)un.tiles.size();
+    G.nr = (int)un.reds.size();
+    G.nslot = un.nslot;
+    slots = std::max(slots, un.nslot);
+    if ((rc = upload_vec(&G.d_t, un.tiles, st))) return rc;
+    if ((rc = upload_vec(&G.d_r, un.reds, st))) return rc;
+  }
+  if (slots > S->part_slots) {
+    hipFree(S->d_part);
+    S->d_part = nullptr;
+    S->part_slots = 0;
+    if ((rc = alloc_bytes(&S->d_part, slots * (int64_t)WT * WT * (int64_t)sizeof(float2), "partial tiles")))
+      return rc;
+    S->part_slots = slots;
+  }
+  S->desired = desired;
+  TSC_HIP_CHECK(hipStreamSynchronize(st));
+  return TSC_OK;
 }
 
 int pk_enqueue(tsc_plan *p, const double2 *in, double2 *out, bool events) {

@@ -63,7 +63,8 @@ class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
                  prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True,
-                 lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True, guessed_f64=False):
+                 lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True, guessed_f64=False,
+                 autotune_flops=1e11):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -92,6 +93,10 @@ class DMRG:
         # iterations, which the FP64 refinement would repeat anyway)
         self.guessed_f64 = guessed_f64
         self._prec_now = prec
+        # superblocks with at least this many flops per H.psi get their work
+        # units tuned (tsc_plan_autotune; the choice is cached per plan shape)
+        self.autotune_flops = autotune_flops
+        self.tuned = {}
         self.sys_trun, self.env_trun = {}, {}   # block length -> wavetrans.Trunc that built it
         self.seed = seed
         self.native_planner = native_planner
@@ -201,6 +206,24 @@ class DMRG:
                 return v
         return self.ctx.vec_from(start_vector(sb.super, self.seed + self.nstep))
 
+    def _autotune(self, plan, psi):
+        """Tune the plan's work units for the precisions this solve uses."""
+        if self.autotune_flops is None or plan.flops() < self.autotune_flops:
+            return 0.0
+        t0 = time.perf_counter()
+        scratch = self.ctx.vec(psi.n)
+        try:
+            precs = {self._prec_now}
+            if self._prec_now != capi.PREC_F64 and (self.refine or self.shadow_f64):
+                precs.add(capi.PREC_F64)
+            for p in sorted(precs):
+                self.tuned[p] = plan.autotune(psi, scratch, p)
+        finally:
+            scratch.close()
+        dt = time.perf_counter() - t0
+        self.timers["autotune"] = self.timers.get("autotune", 0.0) + dt
+        return dt
+
     def _shadow_start(self, psi):
         if not self.shadow_f64 or self._prec_now == capi.PREC_F64:
             return None
@@ -256,6 +279,7 @@ class DMRG:
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         psi = self._initial(sb, guess)
+        t0 += self._autotune(plan, psi)   # reported under "autotune", not "lanczos"
         psi0 = self._shadow_start(psi)
         t1 = time.perf_counter()
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
@@ -323,6 +347,7 @@ class DMRG:
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         psi = self._initial(sb, guess)
+        t0 += self._autotune(plan, psi)   # reported under "autotune", not "lanczos"
         psi0 = self._shadow_start(psi)
         t1 = time.perf_counter()
         E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
