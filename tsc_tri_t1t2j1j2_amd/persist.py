@@ -30,7 +30,6 @@ block still carries, listed in a sidecar JSON file; they restart this
 driver, they are not interchangeable with the reference's model files.
 """
 import json
-import os
 import struct
 
 import numpy as np
