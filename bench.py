@@ -32,10 +32,13 @@ HBM_GBS = 8000.0
 #   bench.py --steps 2 --warmup 1 --no-cpu --no-lanczos --calibrate-copy-mib 1024
 # with --pmc FETCH_SIZE and --pmc WRITE_SIZE; each counter is scaled by its
 # reading for the 1 GiB copy in the same run (FETCH_SIZE x 2.0, WRITE_SIZE
-# x 1.0). Per-dispatch KiB: BF16x3 pk_gemm_kernel 14388044 fetched, 617783
-# written.
+# x 1.0). Per-dispatch KiB of the BF16x3 pk_gemm_kernel for each work-unit
+# target the autotuner may keep: 2048: 14007535 fetched, 622965 written;
+# 4096: 13638127 / 826403; 8192: 12844715 / 1239763.
 MEASURED_TRAFFIC = {
-    ("yc6-m6000", "bf16x3"): (14388044 * 2.0 + 617783 * 1.0) * 1024,
+    ("yc6-m6000", "bf16x3", 2048.0): (14007535 * 2.0 + 622965 * 1.0) * 1024,
+    ("yc6-m6000", "bf16x3", 4096.0): (13638127 * 2.0 + 826403 * 1.0) * 1024,
+    ("yc6-m6000", "bf16x3", 8192.0): (12844715 * 2.0 + 1239763 * 1.0) * 1024,
 }
 
 
@@ -435,7 +438,8 @@ def main():
         dom, kms, kf = "grouped stage 2", k2, f2
     achieved = kf / (kms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.prec]
-    traffic = MEASURED_TRAFFIC.get((args.workload, args.prec)) if dom == "grouped stage 2" else None
+    traffic = (MEASURED_TRAFFIC.get((args.workload, args.prec, tuned.get(args.prec)))
+               if dom == "grouped stage 2" else None)
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 5), "traffic": traffic, "kernel": dom,
             "kernel_ms": round(kms, 3), "flops_per_launch": kf}
