@@ -7,9 +7,14 @@ per step basis -> super basis -> start vector -> Lanczos -> rho ->
 truncation -> renormalisation. Every numerical stage runs through the
 C-ABI (grouped contraction plans, device Lanczos, device rho/eigensolve,
 renormalisation plans); this module only orders the steps and keeps the
-blocks. Start vectors are seeded uniform random numbers (wave_initialize,
-block.f90:1154-1180, made reproducible); the guess transformation of the
-reference (wave_trans_LR/RL) is not used, each Lanczos starts afresh.
+blocks. Inside a half-sweep each Lanczos starts from the previous ground
+state carried into the new superblock (wave_trans_LR/RL,
+dmrg_sub.f90:1471-1789; wavetrans.py); the first step of a half-sweep and
+the warm-up start from seeded uniform random numbers (wave_initialize,
+block.f90:1154-1180, made reproducible). After run(), measure() does the
+final half-sweep and the measurement (Measurement, main.f90:213-354);
+run(checkpoint=..., restart=...) writes and reads the per-length files of
+the reference (persist.py).
 """
 import json
 import os
