@@ -68,3 +68,16 @@ def test_w6j_library_matches_exact_oracle():
         assert abs(got - ref) <= 1e-13 * max(1.0, abs(ref)), (a, got, ref)
         checked += ref != 0.0
     assert checked > 100
+
+
+def test_fortran_module_binds_every_entry_point():
+    """The Fortran binding (the reference's host language) declares an
+    interface for every function of include/tsc_hotpath.h."""
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "tsc_hotpath.h")).read()
+    declared = set(re.findall(r"^(?:int|double|void \*|const char \*)\s*(tsc_[a-z0-9_]+)\(", hdr, re.M))
+    mod = open(os.path.join(root, "tsc_tri_t1t2j1j2_amd", "fortran", "tsc_hotpath_mod.f90")).read()
+    bound = set(re.findall(r'bind\(c, name="(tsc_[a-z0-9_]+)"\)', mod))
+    assert declared and declared <= bound, sorted(declared - bound)

@@ -110,6 +110,151 @@ module tsc_hotpath
       integer(c_int64_t), value :: ori_elems, u_elems, eff_elems
       integer(c_int), value :: prec
     end function
+    integer(c_int) function tsc_version() bind(c, name="tsc_version")
+      import :: c_int
+    end function
+    integer(c_int) function tsc_ctx_sync(ctx) bind(c, name="tsc_ctx_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function tsc_ctx_mfma_layout(ctx, modes3) bind(c, name="tsc_ctx_mfma_layout")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: modes3(3)
+    end function
+    integer(c_int) function tsc_ctx_mem_info(ctx, free_bytes, total_bytes) bind(c, name="tsc_ctx_mem_info")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t) :: free_bytes, total_bytes
+    end function
+    integer(c_int) function tsc_vec_norm(v, r) bind(c, name="tsc_vec_norm")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: v
+      real(c_double) :: r
+    end function
+    function tsc_vec_dptr(v) bind(c, name="tsc_vec_dptr") result(p)
+      import :: c_ptr
+      type(c_ptr), value :: v
+      type(c_ptr) :: p
+    end function
+    integer(c_int) function tsc_vec_fill_gaussian(v, seed, key, sigma) bind(c, name="tsc_vec_fill_gaussian")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: v
+      integer(c_int64_t), value :: seed, key
+      real(c_double), value :: sigma
+    end function
+    integer(c_int) function tsc_vec_axpby(y, a, x, b) bind(c, name="tsc_vec_axpby")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: y, x
+      real(c_double), value :: a, b
+    end function
+    integer(c_int) function tsc_vec_copy(y, x) bind(c, name="tsc_vec_copy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: y, x
+    end function
+    integer(c_int) function tsc_vec_copy_range(y, y_off, x, x_off, n) bind(c, name="tsc_vec_copy_range")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: y, x
+      integer(c_int64_t), value :: y_off, x_off, n
+    end function
+    integer(c_int) function tsc_vec_upload_range(v, off, host, n) bind(c, name="tsc_vec_upload_range")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: v, host
+      integer(c_int64_t), value :: off, n
+    end function
+    integer(c_int) function tsc_vec_download_range(v, off, host, n) bind(c, name="tsc_vec_download_range")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: v, host
+      integer(c_int64_t), value :: off, n
+    end function
+    integer(c_int) function tsc_plan_create_dev(ctx, ops, mid_elems, c1, nc1, o1, no1, c2, nc2, o2, no2, &
+                                                in_elems, out_elems, plan) bind(c, name="tsc_plan_create_dev")
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx, ops, c1, o1, c2, o2
+      integer(c_int64_t), value :: mid_elems, in_elems, out_elems
+      integer(c_int32_t), value :: nc1, no1, nc2, no2
+      type(c_ptr) :: plan
+    end function
+    integer(c_int) function tsc_plan_update_ops(plan, ops) bind(c, name="tsc_plan_update_ops")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan, ops
+    end function
+    integer(c_int) function tsc_plan_fill_ops_gaussian(plan, off, count, seed, key, sigma) &
+        bind(c, name="tsc_plan_fill_ops_gaussian")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: plan
+      integer(c_int64_t), value :: off, count, seed, key
+      real(c_double), value :: sigma
+    end function
+    integer(c_int) function tsc_plan_flops(plan, flops) bind(c, name="tsc_plan_flops")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double) :: flops
+    end function
+    integer(c_int) function tsc_hpsi_host(plan, vin, vout, prec) bind(c, name="tsc_hpsi_host")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan, vin, vout
+      integer(c_int), value :: prec
+    end function
+    integer(c_int) function tsc_hpsi_graph(plan, vin, vout, prec) bind(c, name="tsc_hpsi_graph")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan, vin, vout
+      integer(c_int), value :: prec
+    end function
+    integer(c_int) function tsc_plan_last_times(plan, ms1, ms2) bind(c, name="tsc_plan_last_times")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: plan
+      real(c_float) :: ms1, ms2
+    end function
+    integer(c_int) function tsc_plan_autotune(plan, vin, vout, prec, flags, chosen) bind(c, name="tsc_plan_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan, vin, vout
+      integer(c_int), value :: prec, flags
+      real(c_double) :: chosen
+    end function
+    integer(c_int) function tsc_density_matrix(ctx, psi, nsec, sys_dim, env_dim, side, rho) &
+        bind(c, name="tsc_density_matrix")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx, psi, sys_dim, env_dim, rho
+      integer(c_int32_t), value :: nsec
+      integer(c_int), value :: side
+    end function
+    integer(c_int) function tsc_truncation_operator(ctx, nsec, dim, twos, rho, kept_min, kept_max, trun_err, &
+                                                    eigval, kept, u, scalars) bind(c, name="tsc_truncation_operator")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx, dim, twos, rho, eigval, kept, u, scalars
+      integer(c_int32_t), value :: nsec, kept_min, kept_max
+      real(c_double), value :: trun_err
+    end function
+    integer(c_int) function tsc_superblock_tasks(desc, tasks) bind(c, name="tsc_superblock_tasks")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: desc
+      type(c_ptr) :: tasks
+    end function
+    integer(c_int) function tsc_tasks_info(tasks, counts9) bind(c, name="tsc_tasks_info")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: tasks
+      integer(c_int64_t) :: counts9(9)
+    end function
+    integer(c_int) function tsc_tasks_copy(tasks, c1, o1, c2, o2, super_xy, lsec, rsec) bind(c, name="tsc_tasks_copy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tasks, c1, o1, c2, o2, super_xy, lsec, rsec
+    end function
+    integer(c_int) function tsc_tasks_plan(ctx, tasks, arena, arena_len, plan) bind(c, name="tsc_tasks_plan")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx, tasks, arena
+      integer(c_int64_t), value :: arena_len
+      type(c_ptr) :: plan
+    end function
+    integer(c_int) function tsc_tasks_plan_dev(ctx, tasks, arena, plan) bind(c, name="tsc_tasks_plan_dev")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, tasks, arena
+      type(c_ptr) :: plan
+    end function
+    integer(c_int) function tsc_tasks_destroy(tasks) bind(c, name="tsc_tasks_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tasks
+    end function
     real(c_double) function tsc_w6j(j1, j2, j3, l1, l2, l3) bind(c, name="tsc_w6j")
       import :: c_double, c_int32_t
       integer(c_int32_t), value :: j1, j2, j3, l1, l2, l3
