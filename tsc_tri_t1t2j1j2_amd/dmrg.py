@@ -69,7 +69,7 @@ class DMRG:
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
                  prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True,
                  lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True, guessed_f64=False,
-                 autotune_flops=1e11):
+                 autotune_flops=None):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -99,7 +99,9 @@ class DMRG:
         self.guessed_f64 = guessed_f64
         self._prec_now = prec
         # superblocks with at least this many flops per H.psi get their work
-        # units tuned (tsc_plan_autotune; the choice is cached per plan shape)
+        # units tuned (tsc_plan_autotune; the choice is cached per plan shape);
+        # None: off (along a sweep the plan shapes change from step to step,
+        # so tuning costs more than it saves at M=2000)
         self.autotune_flops = autotune_flops
         self.tuned = {}
         self.sys_trun, self.env_trun = {}, {}   # block length -> wavetrans.Trunc that built it
