@@ -228,8 +228,9 @@ def full_step(ctx, wl, prec, args):
     sysb, envb, s1, s2, bonds, ntot = wl.blocks
     out = {}
     for name, p, rf in (("f64", capi.PREC_F64, 0), ("mixed", prec, args.refine)):
+        # the step's plan has the shape bench tuned above: the cached unit cut is applied
         d = DMRG(ctx, wl.Lx, wl.Ny, t2=wl.t2, kept_min=110, kept_max=wl.M, prec=p, refine=rf, resident=True,
-                 lanczos_budget=(args.lanczos_iters, 0))
+                 lanczos_budget=(args.lanczos_iters, 0), autotune_flops=1e11)
         t0 = time.perf_counter()
         sysd, envd = d._dev(sysb), d._dev(envb)
         ctx.sync()
