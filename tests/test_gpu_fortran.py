@@ -16,4 +16,4 @@ def test_fortran_host_program():
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     lines = [l for l in out.stdout.splitlines() if l.startswith(("PASS", "FAIL"))]
-    assert len(lines) == 3 and all(l.startswith("PASS") for l in lines), out.stdout
+    assert len(lines) == 4 and all(l.startswith("PASS") for l in lines), out.stdout

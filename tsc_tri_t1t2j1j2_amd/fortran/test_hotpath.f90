@@ -1,6 +1,8 @@
 ! Fortran host check of the hot path through iso_c_binding: a Hermitian
 ! block operator psi -> A psi + psi A^T solved by the device Lanczos, a
-! U^H O U renormalisation, and a 6j symbol. Prints PASS/FAIL lines.
+! U^H O U renormalisation, a 6j symbol, and rho / eigensolve / kept-state
+! selection of a two-sector psi (Get_density_matrix_sys +
+! Get_truncation_operator). Prints PASS/FAIL lines.
 program test_hotpath
   use iso_c_binding
   use tsc_hotpath
@@ -14,6 +16,10 @@ program test_hotpath
   integer(c_int32_t) :: info(2)
   integer :: i
   real(c_double) :: re(n, n)
+  complex(c_double_complex), target :: w(10), u(10)
+  integer(c_int32_t), target :: sdim(2), edim(2), twos(2), kept(2)
+  real(c_double), target :: eig(5), scal(4)
+  type(c_ptr) :: wv
 
   ! A = D T D^H with T the tridiagonal (2, -1) Toeplitz matrix and D a
   ! diagonal phase matrix: Hermitian, lowest eigenvalue 2 - 2 cos(pi/(n+1))
@@ -63,6 +69,24 @@ program test_hotpath
   else
     write(*, '(a,f12.8)') 'FAIL w6j ', th
   end if
+  ! psi sectors (sys 3 x env 2) and (2 x 2) with singular values^2 0.5, 0.1 and
+  ! 0.3, 0.1: the two largest states are one per sector, discarded weight 0.2
+  w = (0d0, 0d0)
+  w(1) = sqrt(0.5d0); w(5) = sqrt(0.1d0)
+  w(7) = sqrt(0.3d0); w(10) = sqrt(0.1d0)
+  sdim = [3, 2]; edim = [2, 2]; twos = [0, 0]
+  call tsc_check(tsc_vec_create(ctx, 10_c_int64_t, wv), 'vec_create')
+  call tsc_check(tsc_vec_upload(wv, c_loc(w)), 'vec_upload')
+  call tsc_check(tsc_rho_truncate(ctx, wv, 2_c_int32_t, c_loc(sdim), c_loc(edim), c_loc(twos), 0_c_int, &
+                                  2_c_int32_t, 2_c_int32_t, 1d-12, c_loc(eig), c_loc(kept), c_loc(u), c_loc(scal)), &
+                 'rho_truncate')
+  if (all(kept == [1, 1]) .and. nint(scal(1)) == 2 .and. abs(scal(2) - 0.2d0) < 1d-12 .and. &
+      abs(eig(3) - 0.5d0) < 1d-12 .and. abs(eig(5) - 0.3d0) < 1d-12 .and. abs(abs(u(1)) - 1d0) < 1d-12) then
+    write(*, '(a)') 'PASS rho_truncate'
+  else
+    write(*, '(a,2i4,5f10.5)') 'FAIL rho_truncate ', kept, scal(1), scal(2), eig(3), eig(5), abs(u(1))
+  end if
+  call tsc_check(tsc_vec_destroy(wv), 'vec_destroy')
   call tsc_check(tsc_vec_destroy(psi), 'vec_destroy')
   call tsc_check(tsc_plan_destroy(plan), 'plan_destroy')
   call tsc_check(tsc_ctx_destroy(ctx), 'ctx_destroy')
