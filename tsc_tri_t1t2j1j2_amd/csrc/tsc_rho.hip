@@ -266,16 +266,23 @@ int eig_sectors(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, double2 *d_rho, 
   {
     std::vector<std::thread> pool;
     const int dev = ctx->device;
+    auto feed = [&](int lane) {
+      for (size_t bi : lane_buckets[lane])
+        if ((lane_rc[lane] = run_bucket(bi, lane)) != 0) return;
+    };
     for (int lane = 0; lane < kEigStreams; ++lane) {
       if (lane_buckets[lane].empty()) continue;
-      pool.emplace_back([&, lane, dev]() {
-        if (hipSetDevice(dev) != hipSuccess) {
-          lane_rc[lane] = 1;
-          return;
-        }
-        for (size_t bi : lane_buckets[lane])
-          if ((lane_rc[lane] = run_bucket(bi, lane)) != 0) return;
-      });
+      try {
+        pool.emplace_back([&, lane, dev]() {
+          if (hipSetDevice(dev) != hipSuccess) {
+            lane_rc[lane] = 1;
+            return;
+          }
+          feed(lane);
+        });
+      } catch (...) {
+        feed(lane);  // no thread available: this lane is fed from the calling thread
+      }
     }
     for (auto &t : pool) t.join();
   }

@@ -70,7 +70,9 @@ program test_hotpath
     write(*, '(a,f12.8)') 'FAIL w6j ', th
   end if
   ! psi sectors (sys 3 x env 2) and (2 x 2) with singular values^2 0.5, 0.1 and
-  ! 0.3, 0.1: the two largest states are one per sector, discarded weight 0.2
+  ! 0.3, 0.1; kept_min 1, kept_max 2 (the kept rule adds states past kept_min
+  ! while the discarded weight exceeds trun_err, den_mat.f90:301-316): the two
+  ! largest states, one per sector, discarded weight 0.2
   w = (0d0, 0d0)
   w(1) = sqrt(0.5d0); w(5) = sqrt(0.1d0)
   w(7) = sqrt(0.3d0); w(10) = sqrt(0.1d0)
@@ -78,7 +80,7 @@ program test_hotpath
   call tsc_check(tsc_vec_create(ctx, 10_c_int64_t, wv), 'vec_create')
   call tsc_check(tsc_vec_upload(wv, c_loc(w)), 'vec_upload')
   call tsc_check(tsc_rho_truncate(ctx, wv, 2_c_int32_t, c_loc(sdim), c_loc(edim), c_loc(twos), 0_c_int, &
-                                  2_c_int32_t, 2_c_int32_t, 1d-12, c_loc(eig), c_loc(kept), c_loc(u), c_loc(scal)), &
+                                  1_c_int32_t, 2_c_int32_t, 1d-12, c_loc(eig), c_loc(kept), c_loc(u), c_loc(scal)), &
                  'rho_truncate')
   if (all(kept == [1, 1]) .and. nint(scal(1)) == 2 .and. abs(scal(2) - 0.2d0) < 1d-12 .and. &
       abs(eig(3) - 0.5d0) < 1d-12 .and. abs(eig(5) - 0.3d0) < 1d-12 .and. abs(abs(u(1)) - 1d0) < 1d-12) then
