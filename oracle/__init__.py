@@ -7,7 +7,7 @@ only as the checker. The product (tsc_tri_t1t2j1j2_amd) never imports it.
 
 Pinning (the reference itself cannot be built here: it needs Intel MKL,
 which this image lacks, plus source edits to its compile-time lattice
-parameters; see DESIGN.md):
+parameters):
   wigner.w6j          exact rational Racah sum; checked against sympy's
                       wigner_6j, closed forms and Regge symmetries
   truncation.*        numpy eigh; invariants (Hermitian rho, trace 1,
@@ -19,4 +19,8 @@ parameters):
   synth.hpsi          direct per-term product definition (no reordering)
   su2 / tj_model      exact diagonalisation of small t-J clusters in the
                       U(1) Fock basis (see tests/test_oracle_physics.py)
+  tj_ed correlators   bond sums of <hopping>, <S.S>, <n n> rebuild the ED
+                      energy; sum of <n_i> = N_e; sum of <S_i.S_j> = 0 for a
+                      singlet; pair correlators equal overlaps of annihilated
+                      states; the dimer singlet gives <D+ D> = 1
 """
