@@ -127,3 +127,14 @@ def test_default_measurement_and_files(monkeypatch, tmp_path):
     rows = (tmp_path / "spin_cor.dat").read_text().splitlines()[1:]
     assert [(int(r[:8]), int(r[8:16]), int(r[16:24])) for r in rows] == [(2, 4, 1), (2, 6, 2), (2, 8, 3)]
     assert abs(float(rows[0][24:45]) - out["SS"][(2, 4)]) < 1e-13
+
+
+def test_reference_pair_partners_in_the_env_block():
+    """Partners of Get_mSCOP_Operator_Cor0's env loop (measure.f90:2804-2827)
+    on an 8 x 2 cylinder measured at s1 = 7: xi1 = 4, bond (4, 6) of type 3;
+    the leg sites 10, 12, 14 <= xj1 = 14 lie in the env block (8 sites), the
+    type-1 neighbour does not exist on the last leg of a 2-leg cylinder."""
+    bond, partners = meas.reference_pair_partners(8, 2, 7, 3)
+    assert bond == (4, 6)
+    assert partners == [(10, 3, (10, 12)), (10, 5, (10, 11)), (12, 3, (12, 14)), (12, 5, (12, 13)),
+                        (14, 3, (14, 16)), (14, 5, (14, 15))]
