@@ -444,6 +444,13 @@ def main():
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 5), "traffic": traffic, "kernel": dom,
             "kernel_ms": round(kms, 3), "flops_per_launch": kf}
+    if args.prec == "bf16x3":
+        # every real product is three bf16 MFMA products (hi*hi + hi*lo + lo*hi),
+        # so the path's own ceiling is a third of the bf16 peak
+        roof["mfma_products_per_product"] = 3
+        roof["frac_of_bf16x3_ceiling"] = round(3.0 * achieved / peak, 5)
+    if traffic:
+        roof["traffic_tb_per_s"] = round(traffic / (kms * 1e-3) / 1e12, 3)
 
     extra = {}
     if isinstance(sb, PhysicsWorkload) and not args.no_lanczos:

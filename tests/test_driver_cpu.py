@@ -23,6 +23,20 @@ def test_driver_on_fake_device(monkeypatch, kmax, resident):
     assert (e - e_ed) / (Lx * Ny) < (1e-9 if kmax > 1000 else 1e-4), (e, e_ed)
 
 
+def test_driver_yc6_untruncated_equals_ed(monkeypatch):
+    """Six-leg cylinder (the headline lattice's width, 2 columns, 12 sites,
+    y-periodic bonds of types 1, 3, 5 and the next-nearest ones): with every
+    state kept, warm-up and sweep reproduce the exact ground energy."""
+    ctx = fake_device.install(monkeypatch)
+    Lx, Ny, Ne = 2, 6, 10
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
+    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-10, trun_err=1e-12,
+             native_planner=False, resident=True)
+    e = d.run()
+    assert abs(e - e_ed) / (Lx * Ny) < 1e-10, (e, e_ed)
+
+
 def test_resident_driver_matches_host_driver(monkeypatch):
     """Operators kept in (fake) device arenas, merged and renormalised there,
     give the same DMRG trajectory as host-side blocks."""

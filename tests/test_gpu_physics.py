@@ -93,13 +93,25 @@ def test_truncating_step_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("resident", [True, False])
-@pytest.mark.parametrize("Lx,Ny,Ne", [(3, 3, 8), (2, 4, 6)])
+@pytest.mark.parametrize("Lx,Ny,Ne", [(3, 3, 8), (2, 4, 6), (2, 6, 10)])
 def test_dmrg_untruncated_equals_ed(ctx, Lx, Ny, Ne, resident):
     b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
     e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
     d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-10, resident=resident)
     e = d.run()
     assert abs(e - e_ed) < 1e-8, (e, e_ed)
+
+
+def test_mixed_dmrg_six_leg_equals_ed(ctx):
+    """BF16x3 H.psi with FP64 refinement on the six-leg cylinder (2 columns,
+    every state kept): the energy is the exact one within 1e-8 per site."""
+    Lx, Ny, Ne = 2, 6, 10
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
+    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-10,
+             prec=capi.PREC_BF16X3, refine=2)
+    e = d.run()
+    assert abs(e - e_ed) / (Lx * Ny) < 1e-8, (e, e_ed)
 
 
 def test_resident_dmrg_matches_host_operators(ctx):
