@@ -51,7 +51,7 @@ def parse():
                     choices=["yc6-m6000", "yc6-m12000", "synthetic400", "synthetic-small"])
     ap.add_argument("--no-lanczos", action="store_true", help="H.psi only (no Lanczos, truncation or DMRG check)")
     ap.add_argument("--lanczos-iters", type=int, default=40)
-    ap.add_argument("--refine", type=int, default=2, help="FP64 Lanczos steps after the mixed ones")
+    ap.add_argument("--refine", type=int, default=2, help="steps of the refinement Lanczos pass after the mixed one (its first H.psi FP64)")
     ap.add_argument("--no-dmrg", action="store_true", help="skip the FP64-vs-mixed DMRG energy check")
     ap.add_argument("--no-full-step", action="store_true", help="skip the resident full-step timing")
     ap.add_argument("--dmrg-sweeps", type=int, default=6)
@@ -210,7 +210,7 @@ def lanczos_and_step(ctx, plan, wl, vin, prec, args, hbm_mark=None):
     tr = capi.rho_truncate(ctx, psim, sd, ed, twos, 0, 110, wl.M, 1e-15)
     ttr = time.perf_counter() - t0
     s64, smx = t64 + ttr, tm + ttr
-    return {"lanczos_budget": f"FP64 {it} H.psi; mixed {it} + {refine} FP64 refinement H.psi",
+    return {"lanczos_budget": f"FP64 {it} H.psi; mixed {it} + a {max(2, refine)}-step refinement pass (its first H.psi FP64)",
             "lanczos_f64_s": round(t64, 3), "lanczos_mixed_s": round(tm, 3), "rho_truncate_s": round(ttr, 3),
             "kept": int(tr["num_state"]), "lanczos_rho_s_f64": round(s64, 3), "lanczos_rho_s_mixed": round(smx, 3)}
 

@@ -207,8 +207,10 @@ int tsc_plan_autotune(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precisio
  * Krylov vectors stay resident; alpha/beta/dots are FP64; tridiagonal
  * solves run on the host in FP64. psi holds the start vector on entry and
  * the normalised Ritz vector on exit. precision selects H.psi products
- * for the Krylov build; refine_iters > 0 appends that many FP64 Lanczos
- * steps restarted from the mixed-precision Ritz vector.
+ * for the Krylov build; refine_iters > 0 appends a Lanczos pass of that
+ * many steps (at least 2) restarted from the mixed-precision Ritz vector,
+ * whose first H.psi is FP64 (exact <psi|H psi> and residual) and whose
+ * later products use `precision`.
  * info[0] = total H.psi calls, info[1] = outer restarts used.
  */
 int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer,

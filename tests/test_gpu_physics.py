@@ -49,6 +49,10 @@ def test_lanczos_on_superblock_equals_ed(ctx):
     psi32 = ctx.vec_from(start_vector(psb.super, 7))
     e32, _ = plan.lanczos(psi32, tol=1e-12, prec=capi.PREC_BF16X3, refine_iters=40)
     assert abs(e32 - e_ed) < 1e-9
+    # the default refinement pass (two steps: one FP64 product, one BF16x3)
+    psi2 = ctx.vec_from(start_vector(psb.super, 7))
+    e2, _ = plan.lanczos(psi2, tol=1e-12, prec=capi.PREC_BF16X3, refine_iters=2)
+    assert abs(e2 - e_ed) < 1e-9
 
 
 def test_truncating_step_matches_oracle(ctx):

@@ -763,7 +763,10 @@ struct LanczosResult {
   int restarts;
 };
 
-int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer, int prec, LanczosResult *res) {
+// prec: the first H.psi of each restart (from the start or Ritz vector);
+// prec_rest: the products of the later Krylov vectors
+int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer, int prec, int prec_rest,
+                 LanczosResult *res) {
   tsc_ctx *ctx = p->ctx;
   const int64_t n = psi->n;
   TSC_REQUIRE(max_inner >= 1 && max_inner + 1 <= kMaxVecs, "max_inner out of range");
@@ -858,7 +861,7 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
         uu[last] = nn;
       }
       ++last;
-      LZ_CALL(hpsi_dev(p, V[last - 1], V[last], prec));
+      LZ_CALL(hpsi_dev(p, V[last - 1], V[last], prec_rest));
       ++matvecs;
       {
         const double2 *vv[1] = {V[last - 1]};
@@ -904,12 +907,18 @@ extern "C" int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner,
   TSC_REQUIRE(p && psi && energy, "tsc_lanczos arguments");
   TSC_REQUIRE(max_outer >= 0 && refine_iters >= 0, "tsc_lanczos iteration counts");
   LanczosResult r{0.0, 0, 0};
-  int rc = lanczos_pass(p, psi, tol, max_inner, max_outer, prec, &r);
+  int rc = lanczos_pass(p, psi, tol, max_inner, max_outer, prec, prec, &r);
   if (rc) return rc;
   int mv = r.matvecs, rs = r.restarts;
   if (prec != TSC_PREC_F64 && refine_iters > 0) {
+    // Refinement: a Lanczos pass from the Ritz vector psi whose first
+    // product is FP64. It fixes alpha_1 = <psi|H psi> and the residual
+    // r = H psi - alpha_1 psi (v_2 = r / beta_1) to FP64; the later products
+    // only enter the lowest Ritz value through the weight of v_2.. in it,
+    // O(beta_1^2 / gap^2), so their mixed-precision rounding moves the energy
+    // by O(beta_1^2 / gap^2 * eps * |H|) and they run in the working precision.
     LanczosResult r2{0.0, 0, 0};
-    rc = lanczos_pass(p, psi, tol, std::max(2, refine_iters), 0, TSC_PREC_F64, &r2);
+    rc = lanczos_pass(p, psi, tol, std::max(2, refine_iters), 0, TSC_PREC_F64, prec, &r2);
     if (rc) return rc;
     r.energy = r2.energy;
     mv += r2.matvecs;
