@@ -66,3 +66,18 @@ def test_shadow_error_by_phase(monkeypatch):
     assert len(d.shadow) == len(d.history)
     # the fake device solves every precision in FP64: both solves agree
     assert max(ph.values()) < 1e-12
+
+
+def test_renorm_preparation_failure_reaches_the_caller(monkeypatch):
+    """The renormalisation bookkeeping built beside the solve: an error
+    raised there ends the step with that error, after the solve."""
+    from tsc_tri_t1t2j1j2_amd import hamiltonian as ham
+    ctx = fake_device.install(monkeypatch)
+
+    def broken(*args, **kwargs):
+        raise RuntimeError("renorm bookkeeping failed")
+
+    monkeypatch.setattr(ham, "renorm_transitions", broken)
+    d = DMRG(ctx, 3, 3, Ne=8, kept_min=4, kept_max=12, sweeps=1, lanzero=1e-10, native_planner=False)
+    with pytest.raises(RuntimeError, match="renorm bookkeeping failed"):
+        d.run()

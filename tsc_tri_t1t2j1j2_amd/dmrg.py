@@ -18,6 +18,7 @@ the reference (persist.py).
 """
 import json
 import os
+import threading
 import time
 
 import numpy as np
@@ -356,13 +357,32 @@ class DMRG:
         psi = self._initial(sb, guess)
         t0 += self._autotune(plan, psi)   # reported under "autotune", not "lanczos"
         psi0 = self._shadow_start(psi)
-        t1 = time.perf_counter()
-        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
-                                   max_outer=self.lanczos_budget[1], prec=self._prec_now,
-                                   refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
-        self.timers["lanczos"] += time.perf_counter() - t0
-        self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
-        self._shadow(psi0, plan, tol, E)
+        # the truncation-independent half of the renormalisation plans is host
+        # bookkeeping: a second thread builds it while the solve runs in the
+        # library (ctypes releases the interpreter lock for the call)
+        prepared, failure = {}, []
+
+        def prepare():
+            try:
+                for side, (blk, site, _, keep, pairs) in spec.items():
+                    prepared[side] = ham.renorm_transitions(blk, site, keep, pairs)
+            except BaseException as exc:   # re-raised by the stepping thread
+                failure.append(exc)
+
+        worker = threading.Thread(target=prepare, daemon=True)
+        worker.start()
+        try:
+            t1 = time.perf_counter()
+            E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
+                                       max_outer=self.lanczos_budget[1], prec=self._prec_now,
+                                       refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
+            self.timers["lanczos"] += time.perf_counter() - t0
+            self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
+            self._shadow(psi0, plan, tol, E)
+        finally:
+            worker.join()
+        if failure:
+            raise failure[0]
         self.hpsi_flops += nmv * plan.flops()
         self.hpsi_calls += nmv
         out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
@@ -372,7 +392,8 @@ class DMRG:
             blk, site, to_new, keep, pairs = spec[side]
             U, order, err, ns = self._truncate(sb, psi, idx, self.kept_min, kept_max, trun_err, ident)
             t0 = time.perf_counter()
-            rp = ham.renormalize_plan(blk, site, to_new, keep, U, order, pairs=pairs, resolver=step.offset)
+            rp = ham.renormalize_plan(blk, site, to_new, keep, U, order, pairs=pairs, resolver=step.offset,
+                                      prepared=prepared[side])
             out[side] = resident.renormalize(ctx, step, rp)
             fl, ms = getattr(out[side], "renorm_stats", (0.0, 0.0))
             self.renorm_device[0] += fl

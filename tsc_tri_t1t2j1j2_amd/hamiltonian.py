@@ -384,19 +384,15 @@ def renorm_pairs(blk, bonds_to_new):
     return pairs
 
 
-def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order, pairs=None, resolver=None):
-    """Task arrays computing every operator of blk (x) new_site truncated by U.
-
-    bonds_to_new: [(i, t, J)] with i in blk; keep_sites: sites whose
-    operators the new block carries; U: {enlarged sector x: d_x x kept_x};
-    order: enlarged sectors of the new block, in their new order; pairs: a
-    precomputed renorm_pairs(blk, bonds_to_new); resolver: as in
-    Superblock.plan_builder (resident blocks)."""
+def renorm_transitions(blk, new_site, keep_sites, pairs):
+    """The part of renormalize_plan that does not depend on the truncation:
+    enlarged basis, transitions (embedding factors) of the enlarged block's
+    H and of every carried operator, and their (up_dif, k2). Host-only
+    bookkeeping on the block's structure (no device calls), so a driver
+    may compute it while the superblock is being solved."""
     basis = TotalBasis(blk.sectors)
     so = site_operators()
     # H of the enlarged block: block H + merged block-site bonds
-    if pairs is None:
-        pairs = renorm_pairs(blk, bonds_to_new)
     trans = {"H": scalar_transitions(basis, blk.sectors, ham=blk.ham, pairs=pairs, reduced=True)}
     meta = {"H": (0, 0)}
     for i in keep_sites:
@@ -409,6 +405,23 @@ def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order, pairs=No
                 op = blk.ops[i][name]
             trans[(i, name)] = side_transitions(basis, blk.sectors, spec)
             meta[(i, name)] = (op.up_dif, op.k2)
+    return basis, trans, meta
+
+
+def renormalize_plan(blk, new_site, bonds_to_new, keep_sites, U, order, pairs=None, resolver=None, prepared=None):
+    """Task arrays computing every operator of blk (x) new_site truncated by U.
+
+    bonds_to_new: [(i, t, J)] with i in blk; keep_sites: sites whose
+    operators the new block carries; U: {enlarged sector x: d_x x kept_x};
+    order: enlarged sectors of the new block, in their new order; pairs: a
+    precomputed renorm_pairs(blk, bonds_to_new); resolver: as in
+    Superblock.plan_builder (resident blocks); prepared: the result of
+    renorm_transitions for the same block, site, kept sites and pairs."""
+    if pairs is None:
+        pairs = renorm_pairs(blk, bonds_to_new)
+    if prepared is None:
+        prepared = renorm_transitions(blk, new_site, keep_sites, pairs)
+    basis, trans, meta = prepared
     new_index = {x: n for n, x in enumerate(order)}
     # IN buffer: U_x laid end to end (order of the new block)
     uoff, parts, off = {}, [], 0
