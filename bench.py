@@ -176,8 +176,9 @@ def time_gpu(ctx, plan, vin, vout, prec, steps, warmup):
 def lanczos_and_step(ctx, plan, wl, vin, prec, args, hbm_mark=None):
     """Device time of one DMRG step's hot path on the centre superblock in
     FP64 and in mixed precision: Lanczos at a fixed Krylov budget (FP64: `it`
-    H.psi; mixed: `it` mixed H.psi then `refine` FP64 H.psi restarted from the
-    mixed Ritz vector), then rho / eigensolve / truncation. The sweep time is
+    H.psi; mixed: `it` mixed H.psi, then a refinement pass of `refine` steps
+    restarted from the mixed Ritz vector, whose first H.psi is FP64 and the
+    later ones mixed), then rho / eigensolve / truncation. The sweep time is
     extrapolated from it. The blocks are random, so these Lanczos energies
     are not converged and their difference measures convergence, not
     precision; the precision check is dmrg_precision()."""
@@ -303,8 +304,8 @@ def measured_dmrg(ctx, refine, Lx=8, Ny=4, M=1000, sweeps=2):
 
 
 def dmrg_precision(ctx, sweeps, refine):
-    """|dE|/site of the mixed precision (BF16x3 + `refine` FP64 Lanczos steps
-    per solve) on YC4 4x6 (24 sites, 22 electrons, kept_max 256), two ways:
+    """|dE|/site of the mixed precision (BF16x3, then per solve a refinement
+    pass of `refine` steps whose first H.psi is FP64) on YC4 4x6 (24 sites, 22 electrons, kept_max 256), two ways:
     (1) two whole runs, FP64 and mixed: their difference also contains the
     effect of different truncation choices along the two paths; (2) along the
     mixed run, every superblock is also solved in FP64 from the same start

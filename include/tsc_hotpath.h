@@ -237,7 +237,11 @@ int tsc_density_matrix(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec,
  *   twos[s]        2S of sector s (entropy weights 2S+1)
  *   eigval_out     sum_s d_s eigenvalues, ascending per sector
  *   kept_out[s]    kept states of sector s
- *   u_out          sum_s d_s*kept_s complex128 values
+ *   u_out          sum_s d_s*kept_s complex128 values; u_cap is its
+ *                  size in complex values, checked before anything is
+ *                  written (kept_s can exceed kept_max: the rule keeps
+ *                  kept_min states and then at least one more, and all of
+ *                  them when n <= kept_min; sum_s d_s^2 always suffices)
  *   scalars_out    [0]=num_state [1]=discarded weight [2]=entropy (kept
  *                  spectrum, Entropys) [3]=entropy1 (U(1)-resolved)
  */
@@ -245,7 +249,7 @@ int tsc_truncation_operator(tsc_ctx *ctx, int32_t nsec, const int32_t *dim,
                             const int32_t *twos, const double *rho_c128,
                             int32_t kept_min, int32_t kept_max, double trun_err,
                             double *eigval_out, int32_t *kept_out, double *u_out_c128,
-                            double *scalars_out);
+                            int64_t u_cap, double *scalars_out);
 
 /*
  * Get_density_matrix_sys/_env followed by Get_truncation_operator without
@@ -257,7 +261,20 @@ int tsc_truncation_operator(tsc_ctx *ctx, int32_t nsec, const int32_t *dim,
 int tsc_rho_truncate(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, const int32_t *sys_dim,
                      const int32_t *env_dim, const int32_t *twos, int side, int32_t kept_min,
                      int32_t kept_max, double trun_err, double *eigval_out, int32_t *kept_out,
-                     double *u_out_c128, double *scalars_out);
+                     double *u_out_c128, int64_t u_cap, double *scalars_out);
+
+/*
+ * The kept-state selection of Get_truncation_operator alone
+ * (den_mat.f90:255-364), host code that needs no device: from the
+ * eigenvalues of every sector (ascending per sector, laid end to end) it
+ * sorts the spectrum in the reference's quicksort order and returns the
+ * per-sector kept counts and scalars_out as tsc_truncation_operator does.
+ * kept_min, kept_max and trun_err must be >= 0; the count never passes
+ * the end of the spectrum.
+ */
+int tsc_select_states(int32_t nsec, const int32_t *dim, const int32_t *twos, const double *eigval,
+                      int32_t kept_min, int32_t kept_max, double trun_err, int32_t *kept_out,
+                      double *scalars_out);
 
 /*
  * update_trun_dia / update_trun_ndia (block.f90:3465-3683):

@@ -60,7 +60,9 @@ def select(eig_sectors, twos, kept_min, kept_max, trun_err):
         for i in range(kept_min + 1, n + 1):
             err += ev[i]
         num_state = kept_min
-        while err > trun_err:
+        # the reference's loop has no bound at the end of the spectrum (with
+        # trun_err = 0 a rounding remainder could step past it); stop at n
+        while err > trun_err and num_state < n:
             num_state += 1
             err -= ev[num_state]
             if err <= trun_err or num_state >= kept_max:

@@ -10,9 +10,12 @@ from tsc_tri_t1t2j1j2_amd import capi
 
 
 class FakeVec:
-    def __init__(self, x):
+    def __init__(self, x, live=None):
         self.x = np.array(x, dtype=complex)
         self.n = self.x.size
+        self.live = live
+        if live is not None:
+            live.add(id(self))
 
     def download(self):
         return self.x.copy()
@@ -43,15 +46,21 @@ class FakeVec:
         self.x[:] = x.x
 
     def close(self):
-        pass
+        if self.live is not None:
+            self.live.discard(id(self))
 
 
 class FakeCtx:
+    """Records the vectors and plans that are open (`live`: their ids)."""
+
+    def __init__(self):
+        self.live = set()
+
     def vec_from(self, x):
-        return FakeVec(np.asarray(x).ravel(order="F"))
+        return FakeVec(np.asarray(x).ravel(order="F"), self.live)
 
     def vec(self, n):
-        return FakeVec(np.zeros(int(n), complex))
+        return FakeVec(np.zeros(int(n), complex), self.live)
 
     def sync(self):
         pass
@@ -63,6 +72,9 @@ class FakePlan:
                      c1, o1, c2, o2, mid_elems)
         self.in_elems, self.out_elems = in_elems, out_elems
         self.ops_vec = None
+        self.live = getattr(ctx, "live", None)
+        if self.live is not None:
+            self.live.add(id(self))
 
     @classmethod
     def create_dev(cls, ctx, ops_vec, mid_elems, c1, o1, c2, o2, in_elems, out_elems):
@@ -103,7 +115,8 @@ class FakePlan:
         return 0.0, 0.0
 
     def close(self):
-        pass
+        if self.live is not None:
+            self.live.discard(id(self))
 
 
 def fake_rho_truncate(ctx, psi_vec, sd, ed, twos, side, kmin, kmax, terr):

@@ -81,3 +81,49 @@ def test_renorm_preparation_failure_reaches_the_caller(monkeypatch):
     d = DMRG(ctx, 3, 3, Ne=8, kept_min=4, kept_max=12, sweeps=1, lanzero=1e-10, native_planner=False)
     with pytest.raises(RuntimeError, match="renorm bookkeeping failed"):
         d.run()
+    # the failed step freed its plan, step arena and state vectors: only the
+    # arenas of the two site blocks it was given remain
+    site_arenas = {id(m.arena.vec) for (_, m) in d._dev_cache.values()}
+    assert ctx.live == site_arenas
+
+
+def test_failed_truncation_frees_the_step(monkeypatch):
+    """A failure after the solve (in the truncation) also frees the plan,
+    the step arena, the ground state and the FP64 shadow's start vector."""
+    from tsc_tri_t1t2j1j2_amd import capi
+    ctx = fake_device.install(monkeypatch)
+
+    def broken(*args, **kwargs):
+        raise RuntimeError("eigensolver failed")
+
+    monkeypatch.setattr(capi, "rho_truncate", broken)
+    d = DMRG(ctx, 3, 3, Ne=8, kept_min=1, kept_max=2, sweeps=1, lanzero=1e-10, native_planner=False,
+             prec=capi.PREC_BF16X3, shadow_f64=True)
+    with pytest.raises(RuntimeError, match="eigensolver failed"):
+        d.run()
+    site_arenas = {id(m.arena.vec) for (_, m) in d._dev_cache.values()}
+    assert ctx.live == site_arenas
+
+
+def test_centre_window_repeats_from_the_warm_up_blocks(monkeypatch):
+    """centre_window() runs the first half-sweep's centre steps from the
+    warm-up blocks; a second call from the same blocks (another precision on
+    a device) follows the same steps, and a full run's first half-sweep
+    steps give the same energies. The warm-up with `retain` keeps only the
+    lengths the window needs."""
+    ctx = fake_device.install(monkeypatch)
+    kw = dict(Ne=10, kept_min=10, kept_max=30, sweeps=0, lanzero=1e-11, trun_err=1e-12, native_planner=False)
+    d = DMRG(ctx, 4, 3, retain=3, **kw)
+    d.warm_up()
+    l0 = d.N // 2 - 1
+    assert sorted(L for L in d.sys_blocks if L > 1) == [l0 - 1, l0, l0 + 1]
+    w1 = d.centre_window(2, 1e-10, 30, 1e-12)
+    w2 = d.centre_window(2, 1e-10, 30, 1e-12)
+    assert [r["l"] for r in w1] == [l0, l0 + 1]
+    for a, b in zip(w1, w2):
+        assert abs(a["E"] - b["E"]) < 1e-8 and a["dim"] == b["dim"] and a["kept"] == b["kept"]
+    full = DMRG(ctx, 4, 3, **kw)
+    full.warm_up()
+    full.left_to_right(l0, l0 + 1, 1e-10, 30, 1e-12, "half")
+    for a, h in zip(w1, full.history[-2:]):
+        assert abs(a["E"] - h[2]) < 1e-8 and a["dim"] == h[5]

@@ -27,14 +27,21 @@ def build(verbose=False):
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     objs = []
     os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
+    cmds = []
     for src in SOURCES:
         obj = os.path.join(HERE, "_obj", src.replace(".hip", ".o"))
         cmd = [hipcc, *HIPFLAGS, "-I", os.path.join(ROOT, "include"), "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # the sources compile independently: a few at a time
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(4, len(cmds))) as pool:
+        for r in list(pool.map(lambda c: subprocess.run(c, check=False), cmds)):
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
     cmd = [hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp,
            "-L", os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",

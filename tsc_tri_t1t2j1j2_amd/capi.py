@@ -42,7 +42,7 @@ EXPORTS = [
     "tsc_plan_create", "tsc_plan_create_dev", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
     "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_graph", "tsc_plan_last_times",
     "tsc_plan_autotune",
-    "tsc_lanczos", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate",
+    "tsc_lanczos", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_select_states",
     "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_copy",
     "tsc_tasks_plan", "tsc_tasks_plan_dev", "tsc_tasks_destroy",
 ]
@@ -109,9 +109,11 @@ def lib():
         "tsc_lanczos": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_int, c_int, P(c_double), P(c_int32)]),
         "tsc_density_matrix": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int, c_void_p]),
         "tsc_truncation_operator": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
-                                            c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+                                            c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
         "tsc_rho_truncate": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int, c_int32,
-                                     c_int32, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+                                     c_int32, c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+        "tsc_select_states": (c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_double, c_void_p,
+                                      c_void_p]),
         "tsc_update_trun": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64,
                                     c_int]),
@@ -359,12 +361,13 @@ def density_matrix(ctx, psi_vec, sys_dim, env_dim, side):
     return out
 
 
-def _trunc_outputs(dim, kept_max):
+def _trunc_outputs(dim):
     nsec = len(dim)
     eig = np.empty(int(np.sum(dim)), dtype=np.float64)
     kept = np.empty(nsec, dtype=np.int32)
-    # U may hold at most sum_s d_s * d_s values; bounded by sum d_s * min(d_s, kept_max)
-    ucap = int(np.sum(dim.astype(np.int64) * np.minimum(dim, max(kept_max, 1)).astype(np.int64)))
+    # a sector keeps at most all of its states (the kept rule can pass
+    # kept_max by one and keeps everything when n <= kept_min)
+    ucap = int(np.sum(dim.astype(np.int64) ** 2))
     u = np.empty(max(ucap, 1), dtype=np.complex128)
     scal = np.empty(4, dtype=np.float64)
     return eig, kept, u, scal
@@ -384,9 +387,9 @@ def truncation_operator(ctx, dim, twos, rho, kept_min, kept_max, trun_err):
     dim = np.ascontiguousarray(dim, dtype=np.int32)
     twos = np.ascontiguousarray(twos, dtype=np.int32)
     rho = _c128(rho)
-    eig, kept, u, scal = _trunc_outputs(dim, kept_max)
+    eig, kept, u, scal = _trunc_outputs(dim)
     check(lib().tsc_truncation_operator(ctx.h, len(dim), _ptr(dim), _ptr(twos), _ptr(rho), int(kept_min),
-                                        int(kept_max), float(trun_err), _ptr(eig), _ptr(kept), _ptr(u),
+                                        int(kept_max), float(trun_err), _ptr(eig), _ptr(kept), _ptr(u), u.size,
                                         _ptr(scal)))
     return dict(eigval=eig, kept=kept, U=_split_u(dim, kept, u), num_state=int(scal[0]), trun_err=scal[1],
                 entropy=scal[2], entropy1=scal[3])
@@ -397,12 +400,25 @@ def rho_truncate(ctx, psi_vec, sys_dim, env_dim, twos, side, kept_min, kept_max,
     env_dim = np.ascontiguousarray(env_dim, dtype=np.int32)
     twos = np.ascontiguousarray(twos, dtype=np.int32)
     dim = sys_dim if side == 0 else env_dim
-    eig, kept, u, scal = _trunc_outputs(dim, kept_max)
+    eig, kept, u, scal = _trunc_outputs(dim)
     check(lib().tsc_rho_truncate(ctx.h, psi_vec.h, len(dim), _ptr(sys_dim), _ptr(env_dim), _ptr(twos), int(side),
                                  int(kept_min), int(kept_max), float(trun_err), _ptr(eig), _ptr(kept), _ptr(u),
-                                 _ptr(scal)))
+                                 u.size, _ptr(scal)))
     return dict(eigval=eig, kept=kept, U=_split_u(dim, kept, u), num_state=int(scal[0]), trun_err=scal[1],
                 entropy=scal[2], entropy1=scal[3])
+
+
+def select_states(dim, twos, eigval, kept_min, kept_max, trun_err):
+    """The kept-state rule alone (host code of the library, no device)."""
+    dim = np.ascontiguousarray(dim, dtype=np.int32)
+    twos = np.ascontiguousarray(twos, dtype=np.int32)
+    eigval = np.ascontiguousarray(eigval, dtype=np.float64)
+    assert eigval.size == int(np.sum(dim))
+    kept = np.empty(len(dim), dtype=np.int32)
+    scal = np.empty(4, dtype=np.float64)
+    check(lib().tsc_select_states(len(dim), _ptr(dim), _ptr(twos), _ptr(eigval), int(kept_min), int(kept_max),
+                                  float(trun_err), _ptr(kept), _ptr(scal)))
+    return dict(kept=kept, num_state=int(scal[0]), trun_err=scal[1], entropy=scal[2], entropy1=scal[3])
 
 
 def update_trun(ctx, blocks, prec=PREC_F64):

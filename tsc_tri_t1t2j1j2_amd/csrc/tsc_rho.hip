@@ -367,7 +367,10 @@ void select_states(int32_t nsec, const int32_t *dim, const int32_t *twos, const 
   } else {
     for (int i = kept_min + 1; i <= n; ++i) err += ev[i];
     num_state = kept_min;
-    while (err > trun_err) {
+    // the reference's loop (den_mat.f90:306-314) has no bound at the end of
+    // the spectrum: with trun_err = 0 a rounding remainder of err could step
+    // past it, so the count stops at n
+    while (err > trun_err && num_state < n) {
       ++num_state;
       err -= ev[num_state];
       if (err <= trun_err || num_state >= kept_max) break;
@@ -386,7 +389,7 @@ void select_states(int32_t nsec, const int32_t *dim, const int32_t *twos, const 
 
 int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_t *twos, double2 *d_rho,
                     const std::vector<int64_t> &rho_off, int32_t kept_min, int32_t kept_max, double trun_err,
-                    double *eigval_out, int32_t *kept_out, double *u_out, double *scalars_out) {
+                    double *eigval_out, int32_t *kept_out, double *u_out, int64_t u_cap, double *scalars_out) {
   std::vector<int64_t> w_off(nsec);
   int64_t wt = 0;
   for (int32_t s = 0; s < nsec; ++s) {
@@ -401,6 +404,13 @@ int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_
   TSC_HIP_CHECK(hipMemcpy(w.data(), dw.p, sizeof(double) * wt, hipMemcpyDeviceToHost));
   select_states(nsec, dim, twos, w.data(), kept_min, kept_max, trun_err, kept_out, scalars_out);
   if (eigval_out) std::memcpy(eigval_out, w.data(), sizeof(double) * wt);
+  int64_t u_need = 0;
+  for (int32_t s = 0; s < nsec; ++s) u_need += (int64_t)dim[s] * kept_out[s];
+  if (u_need > u_cap) {
+    set_error("u_out holds " + std::to_string(u_cap) + " complex values; the kept eigenvectors need " +
+              std::to_string(u_need) + " (sum over sectors of dim * kept; sum dim^2 always suffices)");
+    return TSC_EINVAL;
+  }
   // U_s = last kept_s eigenvector columns of sector s (den_mat.f90:392-398)
   int64_t uoff = 0;
   for (int32_t s = 0; s < nsec; ++s) {
@@ -442,9 +452,11 @@ extern "C" int tsc_density_matrix(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec
 
 extern "C" int tsc_truncation_operator(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_t *twos,
                                        const double *rho, int32_t kept_min, int32_t kept_max, double trun_err,
-                                       double *eigval_out, int32_t *kept_out, double *u_out, double *scalars_out) {
+                                       double *eigval_out, int32_t *kept_out, double *u_out, int64_t u_cap,
+                                       double *scalars_out) {
   TSC_REQUIRE(ctx && (nsec == 0 || (dim && twos && rho)) && kept_out && u_out && scalars_out,
               "tsc_truncation_operator arguments");
+  TSC_REQUIRE(kept_min >= 0 && kept_max >= 0 && trun_err >= 0.0, "kept_min, kept_max and trun_err must be >= 0");
   int64_t rho_elems = 0;
   std::vector<int64_t> off(nsec);
   for (int32_t s = 0; s < nsec; ++s) {
@@ -456,15 +468,16 @@ extern "C" int tsc_truncation_operator(tsc_ctx *ctx, int32_t nsec, const int32_t
   TSC_HIP_CHECK(hipMalloc(&drho.p, sizeof(double2) * std::max<int64_t>(rho_elems, 1)));
   TSC_HIP_CHECK(hipMemcpy(drho.p, rho, sizeof(double2) * rho_elems, hipMemcpyHostToDevice));
   return truncate_device(ctx, nsec, dim, twos, (double2 *)drho.p, off, kept_min, kept_max, trun_err, eigval_out,
-                         kept_out, u_out, scalars_out);
+                         kept_out, u_out, u_cap, scalars_out);
 }
 
 extern "C" int tsc_rho_truncate(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, const int32_t *sys_dim,
                                 const int32_t *env_dim, const int32_t *twos, int side, int32_t kept_min,
                                 int32_t kept_max, double trun_err, double *eigval_out, int32_t *kept_out,
-                                double *u_out, double *scalars_out) {
+                                double *u_out, int64_t u_cap, double *scalars_out) {
   TSC_REQUIRE(ctx && psi && (nsec == 0 || (sys_dim && env_dim && twos)) && kept_out && u_out && scalars_out,
               "tsc_rho_truncate arguments");
+  TSC_REQUIRE(kept_min >= 0 && kept_max >= 0 && trun_err >= 0.0, "kept_min, kept_max and trun_err must be >= 0");
   TSC_REQUIRE(side == 0 || side == 1, "side must be 0 (sys) or 1 (env)");
   std::vector<int32_t> dim(nsec);
   int64_t rho_elems = 0;
@@ -478,7 +491,18 @@ extern "C" int tsc_rho_truncate(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, 
   int rc = build_rho(ctx, psi->d, psi->n, nsec, sys_dim, env_dim, side, (double2 *)rho.p, rho_elems, off);
   if (rc) return rc;
   return truncate_device(ctx, nsec, dim.data(), twos, (double2 *)rho.p, off, kept_min, kept_max, trun_err,
-                         eigval_out, kept_out, u_out, scalars_out);
+                         eigval_out, kept_out, u_out, u_cap, scalars_out);
+}
+
+extern "C" int tsc_select_states(int32_t nsec, const int32_t *dim, const int32_t *twos, const double *eigval,
+                                 int32_t kept_min, int32_t kept_max, double trun_err, int32_t *kept_out,
+                                 double *scalars_out) {
+  TSC_REQUIRE(nsec >= 0 && (nsec == 0 || (dim && twos && eigval)) && kept_out && scalars_out,
+              "tsc_select_states arguments");
+  TSC_REQUIRE(kept_min >= 0 && kept_max >= 0 && trun_err >= 0.0, "kept_min, kept_max and trun_err must be >= 0");
+  for (int32_t s = 0; s < nsec; ++s) TSC_REQUIRE(dim[s] >= 0 && twos[s] >= 0, "negative dim / 2S");
+  select_states(nsec, dim, twos, eigval, kept_min, kept_max, trun_err, kept_out, scalars_out);
+  return TSC_OK;
 }
 
 extern "C" int tsc_update_trun(tsc_ctx *ctx, int32_t nblk, const int32_t *row, const int32_t *col, const int32_t *kl,

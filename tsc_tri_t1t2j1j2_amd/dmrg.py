@@ -68,9 +68,9 @@ def start_vector(super_basis, seed):
 class DMRG:
     def __init__(self, ctx, Lx, Ny, t1=1.0, t2=0.2, J1=0.4, J2=None, Ne=None, delta=12, S2=0,
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
-                 prec=capi.PREC_F64, refine=0, seed=1234, log=None, native_planner=True, resident=True,
+                 prec=capi.PREC_F64, refine=None, seed=1234, log=None, native_planner=True, resident=True,
                  lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True, guessed_f64=False,
-                 autotune_flops=None):
+                 autotune_flops=None, retain=None):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -83,7 +83,11 @@ class DMRG:
         self.sweeps = sweeps
         self.lanzero = lanzero
         self.trun_err = trun_err
-        self.prec, self.refine = prec, refine
+        # refine: steps of the refinement pass after each mixed-precision
+        # solve (its first H.psi FP64; tsc_lanczos refine_iters); None gives
+        # 2 for a mixed precision and 0 for FP64, 0 turns the pass off
+        self.prec = prec
+        self.refine = (2 if prec != capi.PREC_F64 else 0) if refine is None else int(refine)
         # (inner iterations, restarts) of each solve: lanczos_memo_new's loop_in=100, loop_out=4 (eigen.f90:14)
         self.lanczos_budget = tuple(int(v) for v in lanczos_budget)
         # shadow_f64: with a mixed precision, also solve every superblock in
@@ -105,6 +109,11 @@ class DMRG:
         # so tuning costs more than it saves at M=2000)
         self.autotune_flops = autotune_flops
         self.tuned = {}
+        # retain: during the warm-up keep only the blocks of the last `retain`
+        # lengths on each side (None: keep all, as the sweeps need them); for
+        # runs that stop at the centre, whose stored blocks would not fit in
+        # HBM at large M
+        self.retain = retain
         self.sys_trun, self.env_trun = {}, {}   # block length -> wavetrans.Trunc that built it
         self.seed = seed
         self.native_planner = native_planner
@@ -242,10 +251,12 @@ class DMRG:
     def _shadow(self, psi0, plan, tol, E):
         if psi0 is None:
             return
+        t0 = time.perf_counter()
         E64, _ = plan.lanczos(psi0, tol=tol, max_inner=self.lanczos_budget[0], max_outer=self.lanczos_budget[1],
                               prec=capi.PREC_F64)
         psi0.close()
         self.shadow.append((self.nstep, E, E64))
+        self.timers["shadow"] = self.timers.get("shadow", 0.0) + time.perf_counter() - t0
 
     def shadow_by_phase(self):
         """Largest same-superblock |E_mixed - E_FP64| / site per phase of the
@@ -272,10 +283,21 @@ class DMRG:
         """One DMRG step; grow: 'sys', 'env' or 'both'. Returns the energy,
         the grown block(s) with the truncations that built them, and the
         ground state (device vector) with its superblock bases. guess(sb):
-        Lanczos start vector for superblock sb, or None."""
+        Lanczos start vector for superblock sb, or None. out["seconds"] is
+        the step's wall time without the FP64 shadow solve."""
+        t_step, sh0 = time.perf_counter(), self.timers.get("shadow", 0.0)
         if self.resident:
-            return self._step_resident(sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err,
-                                       identity_sys, identity_env, partners, guess)
+            out = self._step_resident(sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err,
+                                      identity_sys, identity_env, partners, guess)
+        else:
+            out = self._step_host(sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys,
+                                  identity_env, partners, guess)
+        out["seconds"] = time.perf_counter() - t_step - (self.timers.get("shadow", 0.0) - sh0)
+        return out
+
+    def _step_host(self, sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys,
+                   identity_env, partners, guess):
+        """step() with the block operators uploaded with each plan."""
         t0 = time.perf_counter()
         sb = ham.Superblock(sys, env, s1, s2, bonds, ntot, self.S2)
         if self.native_planner:
@@ -341,6 +363,30 @@ class DMRG:
                 keep = self._keep_sites(blk.sites + [site], partners)
                 spec[side] = (blk, site, to_new, keep, ham.renorm_pairs(blk, to_new))
         step = resident.StepArena(ctx, [sysd, envd])
+        made = {}   # device objects of this step, recorded as they appear
+        try:
+            out = self._solve_and_grow(ctx, sb, step, terms, spec, guess, tol, kept_max, trun_err, identity_sys,
+                                       identity_env, sys, t0, made)
+        except BaseException:
+            # a failed step releases what it allocated on the device
+            for key in ("psi0", "psi", "sys", "env"):
+                obj = made.get(key)
+                obj = getattr(obj, "arena", obj)
+                if obj is not None:
+                    obj.close()
+            raise
+        finally:
+            if made.get("plan") is not None:
+                made["plan"].close()
+            step.close()
+        self.nstep += 1
+        return out
+
+    def _solve_and_grow(self, ctx, sb, step, terms, spec, guess, tol, kept_max, trun_err, identity_sys,
+                        identity_env, sys, t0, made):
+        """The resident step once its StepArena exists: merges, plan, start
+        vector, Lanczos, truncation and renormalisation. Device objects go
+        into `made` (plan, psi0, psi, sys, env) as they are created."""
         step.add_merges(terms)
         for v in spec.values():
             step.add_merges(v[4])
@@ -352,11 +398,12 @@ class DMRG:
         else:
             plan = sb.plan_builder(resolver=step.offset, terms=terms).build_dev(ctx, step.vec, sb.super.dim,
                                                                                 sb.super.dim)
+        made["plan"] = plan
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        psi = self._initial(sb, guess)
+        psi = made["psi"] = self._initial(sb, guess)
         t0 += self._autotune(plan, psi)   # reported under "autotune", not "lanczos"
-        psi0 = self._shadow_start(psi)
+        psi0 = made["psi0"] = self._shadow_start(psi)
         # the truncation-independent half of the renormalisation plans is host
         # bookkeeping: a second thread builds it while the solve runs in the
         # library (ctypes releases the interpreter lock for the call)
@@ -378,6 +425,7 @@ class DMRG:
                                        refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
             self.timers["lanczos"] += time.perf_counter() - t0
             self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
+            made["psi0"] = None   # _shadow closes it
             self._shadow(psi0, plan, tol, E)
         finally:
             worker.join()
@@ -385,7 +433,7 @@ class DMRG:
             raise failure[0]
         self.hpsi_flops += nmv * plan.flops()
         self.hpsi_calls += nmv
-        out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num}
+        out = {"E": E, "matvecs": nmv, "dim": sb.super.dim, "nsec": sb.super.num, "psi": psi}
         for side, idx, ident in (("sys", 0, identity_sys), ("env", 1, identity_env)):
             if side not in spec:
                 continue
@@ -394,18 +442,14 @@ class DMRG:
             t0 = time.perf_counter()
             rp = ham.renormalize_plan(blk, site, to_new, keep, U, order, pairs=pairs, resolver=step.offset,
                                       prepared=prepared[side])
-            out[side] = resident.renormalize(ctx, step, rp)
+            out[side] = made[side] = resident.renormalize(ctx, step, rp)
             fl, ms = getattr(out[side], "renorm_stats", (0.0, 0.0))
             self.renorm_device[0] += fl
             self.renorm_device[1] += ms
             out[side + "_trun"] = wavetrans.Trunc(sb.sys_bs if idx == 0 else sb.env_bs, U, order)
             self.timers["renorm"] += time.perf_counter() - t0
             out[side + "_err"], out[side + "_kept"] = err, ns
-        plan.close()
-        out["psi"] = psi
         out["bases"] = SimpleNamespace(super=sb.super, sys_bs=sb.sys_bs, env_bs=sb.env_bs)
-        step.close()
-        self.nstep += 1
         return out
 
     @staticmethod
@@ -463,8 +507,72 @@ class DMRG:
             r["psi"].close()
             self.history.append(("warmup", l, r["E"], r.get("sys_err", 0.0), r.get("sys_kept", 0), r["dim"]))
             self._save_state()
-            self._say(f"warm-up l={l} N_e={ntot} E={r['E']:.12f} dim={r['dim']} mv={r['matvecs']}")
+            self._say(f"warm-up l={l} N_e={ntot} E={r['E']:.12f} dim={r['dim']} nsec={r['nsec']} "
+                      f"mv={r['matvecs']} {r['seconds']:.2f}s")
+            if self.retain:
+                self._drop_blocks(l + 2 - self.retain)
             label += 1
+
+    def _drop_blocks(self, below):
+        """Free the stored blocks (and their truncations) of lengths 2 ..
+        below-1 on both sides."""
+        for blocks, truns in ((self.sys_blocks, self.sys_trun), (self.env_blocks, self.env_trun)):
+            for L in [L for L in blocks if 1 < L < below]:
+                arena = getattr(blocks.pop(L), "arena", None)
+                if arena is not None:
+                    arena.close()
+                truns.pop(L, None)
+
+    def centre_window(self, nsteps, tol, kept_max, trun_err, tag="window"):
+        """After warm_up(): `nsteps` left-to-right sweep steps from the centre
+        (l = N/2-1, ...), as the first half-sweep runs them (the first from
+        a random start, the others from the wave-function guess). The blocks
+        they grow are dropped afterwards, so the window can be run again from
+        the same warm-up blocks in another precision. Returns one record per
+        step: energy, wall seconds (without FP64 shadow solves), H.psi calls,
+        superblock dimension and sectors, kept states, discarded weight."""
+        N = self.N
+        l0 = N // 2 - 1
+        if self.sys_blocks.get(l0) is None or any(self.env_blocks.get(N - l - 2) is None
+                                                   for l in range(l0, l0 + nsteps)):
+            raise ValueError("the centre window needs the warm-up blocks of lengths N/2-1 and below")
+        saved = {L: (self.sys_blocks.get(L), self.sys_trun.get(L)) for L in range(l0 + 1, l0 + nsteps + 1)}
+        recs = []
+        prev = None
+        try:
+            for l in range(l0, l0 + nsteps):
+                guess = self._guess("LR", prev, self.sys_trun.get(l), self.env_trun.get(N - l - 1))
+                sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
+                r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err,
+                              guess=guess)
+                if l + 1 in self.sys_blocks and self.sys_blocks[l + 1] is not saved[l + 1][0]:
+                    old = getattr(self.sys_blocks[l + 1], "arena", None)
+                    if old is not None:
+                        old.close()
+                self.sys_blocks[l + 1], self.sys_trun[l + 1] = r["sys"], r["sys_trun"]
+                if prev is not None:
+                    prev[0].close()
+                prev = (r["psi"], r["bases"])
+                recs.append({"l": l, "E": r["E"], "seconds": r["seconds"], "matvecs": r["matvecs"],
+                             "dim": r["dim"], "nsec": r["nsec"], "kept": r["sys_kept"], "trun_err": r["sys_err"]})
+                self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
+                self._say(f"{tag} l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
+                          f"dim={r['dim']} nsec={r['nsec']} mv={r['matvecs']} {r['seconds']:.2f}s")
+        finally:
+            if prev is not None:
+                prev[0].close()
+            for L, (blk, tr) in saved.items():
+                cur = self.sys_blocks.get(L)
+                if cur is not None and cur is not blk:
+                    arena = getattr(cur, "arena", None)
+                    if arena is not None:
+                        arena.close()
+                if blk is None:
+                    self.sys_blocks.pop(L, None)
+                    self.sys_trun.pop(L, None)
+                else:
+                    self.sys_blocks[L], self.sys_trun[L] = blk, tr
+        return recs
 
     # ---- finite sweeps -------------------------------------------------
     def _sweep_step_sys(self, l, tol, kept_max, trun_err, guess=None):

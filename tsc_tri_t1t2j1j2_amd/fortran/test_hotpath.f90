@@ -80,7 +80,8 @@ program test_hotpath
   call tsc_check(tsc_vec_create(ctx, 10_c_int64_t, wv), 'vec_create')
   call tsc_check(tsc_vec_upload(wv, c_loc(w)), 'vec_upload')
   call tsc_check(tsc_rho_truncate(ctx, wv, 2_c_int32_t, c_loc(sdim), c_loc(edim), c_loc(twos), 0_c_int, &
-                                  1_c_int32_t, 2_c_int32_t, 1d-12, c_loc(eig), c_loc(kept), c_loc(u), c_loc(scal)), &
+                                  1_c_int32_t, 2_c_int32_t, 1d-12, c_loc(eig), c_loc(kept), c_loc(u), 10_c_int64_t, &
+                                  c_loc(scal)), &
                  'rho_truncate')
   if (all(kept == [1, 1]) .and. nint(scal(1)) == 2 .and. abs(scal(2) - 0.2d0) < 1d-12 .and. &
       abs(eig(3) - 0.5d0) < 1d-12 .and. abs(eig(5) - 0.3d0) < 1d-12 .and. abs(abs(u(1)) - 1d0) < 1d-12) then

@@ -94,11 +94,19 @@ module tsc_hotpath
       integer(c_int32_t) :: info(2)
     end function
     integer(c_int) function tsc_rho_truncate(ctx, psi, nsec, sys_dim, env_dim, twos, side, kept_min, kept_max, &
-                                             trun_err, eigval, kept, u, scalars) bind(c, name="tsc_rho_truncate")
-      import :: c_int, c_ptr, c_int32_t, c_double
+                                             trun_err, eigval, kept, u, u_cap, scalars) bind(c, name="tsc_rho_truncate")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
       type(c_ptr), value :: ctx, psi, sys_dim, env_dim, twos, eigval, kept, u, scalars
       integer(c_int32_t), value :: nsec, kept_min, kept_max
       integer(c_int), value :: side
+      real(c_double), value :: trun_err
+      integer(c_int64_t), value :: u_cap
+    end function
+    integer(c_int) function tsc_select_states(nsec, dim, twos, eigval, kept_min, kept_max, trun_err, kept, &
+                                              scalars) bind(c, name="tsc_select_states")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: dim, twos, eigval, kept, scalars
+      integer(c_int32_t), value :: nsec, kept_min, kept_max
       real(c_double), value :: trun_err
     end function
     integer(c_int) function tsc_update_trun(ctx, nblk, row, col, kl, kr, ori_off, ul_off, ur_off, eff_off, &
@@ -220,11 +228,13 @@ module tsc_hotpath
       integer(c_int), value :: side
     end function
     integer(c_int) function tsc_truncation_operator(ctx, nsec, dim, twos, rho, kept_min, kept_max, trun_err, &
-                                                    eigval, kept, u, scalars) bind(c, name="tsc_truncation_operator")
-      import :: c_int, c_ptr, c_int32_t, c_double
+                                                    eigval, kept, u, u_cap, scalars) &
+                                                    bind(c, name="tsc_truncation_operator")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
       type(c_ptr), value :: ctx, dim, twos, rho, eigval, kept, u, scalars
       integer(c_int32_t), value :: nsec, kept_min, kept_max
       real(c_double), value :: trun_err
+      integer(c_int64_t), value :: u_cap
     end function
     integer(c_int) function tsc_superblock_tasks(desc, tasks) bind(c, name="tsc_superblock_tasks")
       import :: c_int, c_ptr

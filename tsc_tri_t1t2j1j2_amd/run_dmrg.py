@@ -49,8 +49,20 @@ def main(argv=None):
                          "from the blocks in --checkpoint, 4 only measures the state stored there")
     ap.add_argument("--log-dir", default=None, metavar="DIR",
                     help="append Energy_All.dat, Kept_State.dat and entropy1.dat lines per step to DIR")
+    ap.add_argument("--window", type=int, default=0, metavar="K",
+                    help="run only the warm-up to the centre, then K left-to-right sweep steps from the centre "
+                         "in each precision of --window-precs, every one from the same warm-up blocks; print "
+                         "per-step records (blocks of older lengths are freed during the warm-up)")
+    ap.add_argument("--window-precs", default="bf16x3,f64",
+                    help="precisions of the centre window; a mixed one is also solved in FP64 from the same "
+                         "start vectors (same-path energy error)")
+    ap.add_argument("--dump-centre", default=None, metavar="FILE",
+                    help="with --window: write the sector lists (N, 2S, multiplets) of the two centre blocks "
+                         "after the warm-up, and the window records, to FILE (JSON)")
     a = ap.parse_args(argv)
     ctx = capi.Context(0)
+    if a.window:
+        return centre_window(ctx, a)
     restart = None
     if a.restart:
         from . import persist
@@ -104,6 +116,53 @@ def main(argv=None):
         out["measure_files"] = meas.write_reference_files(res, a.measure, a.Lx, a.Ny, d.J2, d.kept_min,
                                                           d.kept_maxf, d.Ne)
         out["E_measured_state"] = res["energy"]
+    print(json.dumps(out), flush=True)
+
+
+def centre_window(ctx, a):
+    """Warm-up to the centre, then the same K centre steps per precision."""
+    t0 = time.perf_counter()
+
+    def log(msg):
+        print(f"[{time.perf_counter() - t0:8.1f}s] {msg}", file=sys.stderr, flush=True)
+
+    precs = [p.strip() for p in a.window_precs.split(",") if p.strip()]
+    p0 = capi.PREC_NAMES[precs[0]]
+    d = DMRG(ctx, a.Lx, a.Ny, t1=a.t1, t2=a.t2, J1=a.J1, J2=a.J2, delta=a.delta, kept_min=a.kept_min,
+             kept_max=a.kept_max, sweeps=0, lanzero=a.lanzero, trun_err=a.trun_err, prec=p0,
+             refine=a.refine if p0 != capi.PREC_F64 else 0, seed=a.seed, log=log, retain=a.window + 1)
+    d.warm_up()
+    t_warm = time.perf_counter() - t0
+    l0 = d.N // 2 - 1
+    out = {"lattice": f"YC{a.Ny} {a.Ny}x{a.Lx}", "sites": d.N, "electrons": d.Ne, "kept_max": a.kept_max,
+           "couplings": {"t1": d.t1, "t2": d.t2, "J1": d.J1, "J2": d.J2}, "warmup_precision": precs[0],
+           "warmup_s": round(t_warm, 2), "warmup_timers_s": {k: round(v, 2) for k, v in d.timers.items()},
+           "centre_sys_len": l0, "centre_sys_sectors": [list(map(int, x)) for x in d.sys_blocks[l0].sectors],
+           "centre_env_sectors": [list(map(int, x)) for x in d.env_blocks[l0].sectors],
+           "centre_sys_sites_kept": sorted(int(i) for i in d.sys_blocks[l0].ops),
+           "centre_env_sites_kept": sorted(int(i) for i in d.env_blocks[l0].ops), "windows": {}}
+    if a.dump_centre:
+        with open(a.dump_centre, "w") as fh:
+            json.dump(out, fh)
+    lanerr = a.lanzero
+    tol, terr = lanerr * 15, a.trun_err * 15   # the first half-sweep's tolerances (main.f90:124-131)
+    for name in precs:
+        p = capi.PREC_NAMES[name]
+        d.prec, d.refine = p, (a.refine if p != capi.PREC_F64 else 0)
+        d.shadow_f64, d.shadow = (p != capi.PREC_F64), []
+        t1 = dict(d.timers)
+        recs = d.centre_window(a.window, tol, a.kept_max, terr, tag=f"window-{name}")
+        w = {"steps": recs, "seconds": round(sum(r["seconds"] for r in recs), 3),
+             "timers_s": {k: round(v - t1.get(k, 0.0), 3) for k, v in d.timers.items()}}
+        if d.shadow:
+            de = [abs(em - e64) / d.N for (_, em, e64) in d.shadow]
+            w["same_path_dE_per_site"] = de
+            w["same_path_max_dE_per_site"] = max(de)
+        out["windows"][name] = w
+        log(f"window {name}: {w['seconds']:.2f}s {w['timers_s']}")
+        if a.dump_centre:
+            with open(a.dump_centre, "w") as fh:
+                json.dump(out, fh)
     print(json.dumps(out), flush=True)
 
 
