@@ -183,6 +183,15 @@ int tsc_plan_flops(const tsc_plan *p, double *flops);
 int tsc_hpsi(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precision);
 /* same with host arrays (uploads, runs, downloads) */
 int tsc_hpsi_host(tsc_plan *p, const double *in_c128, double *out_c128, int precision);
+/*
+ * out = H in in FP64 as one rocBLAS ZGEMM call per contribution, output
+ * block by output block, on the context stream: the reference's own
+ * per-product structure (model_wave's ZGEMM per operator pair and sector,
+ * get_sys.f90:1783-5204), kept as the baseline the grouped kernels are
+ * measured against. Plans with elementwise terms (TSC_OP_I) or
+ * TSC_OP_CONJ operands are rejected with TSC_EINVAL.
+ */
+int tsc_hpsi_loop(tsc_plan *p, const tsc_vec *in, tsc_vec *out);
 /* capture one application in a hipGraph and replay it from then on */
 int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precision);
 /* device time (ms) of the last stage-1 / stage-2 launches, HIP events */
@@ -351,6 +360,17 @@ int tsc_tasks_info(const tsc_tasks *t, int64_t *counts9);
  * lsec/rsec: (N, 2J, dim) of the enlarged sys/env bases */
 int tsc_tasks_copy(const tsc_tasks *t, tsc_contrib *c1, tsc_outblock *o1, tsc_contrib *c2,
                    tsc_outblock *o2, int32_t *super_xy, int32_t *lsec, int32_t *rsec);
+/*
+ * flops2[0] = flops of one application as planned (8 per complex MAC, each
+ * stage-1 intermediate once; the planner forms O_L psi or psi O_R^T first,
+ * per source window and term, whichever is cheaper with the window's
+ * intermediates shared); flops2[1] = the algorithmic count with every
+ * (L transition, R transition, source window) product charged at its own
+ * cheaper order and nothing shared. The reference's block-block kernel
+ * chooses the order per product by shape in the same way
+ * (get_sys.f90:3385-3408).
+ */
+int tsc_tasks_flops(const tsc_tasks *t, double *flops2);
 int tsc_tasks_plan(tsc_ctx *ctx, const tsc_tasks *t, const double *arena_c128, int64_t arena_len,
                    tsc_plan **plan);
 /* the same plan over an operator arena already on the device (borrowed) */

@@ -102,3 +102,24 @@ def timed_sample(ops, c1, o1, c2, o2, mid_len, vin, seconds, seed=0):
         if elapsed >= seconds:
             break
     return flops, elapsed, nwin
+
+
+def execute_windows(ops, c1, o1, c2, o2, mid_len, vin, which):
+    """Stage-2 output windows o2[which] only, with the stage-1 intermediates
+    they read; returns {window index: m x n array}."""
+    mid = np.zeros(max(mid_len, 1), complex)
+    bufs = {0: ops, 1: vin, 2: mid, 3: None}
+    mid_start = o1["c_off"] if len(o1) else np.zeros(0, np.int64)
+    done = set()
+    out = {}
+    for w in which:
+        ob = o2[int(w)]
+        for c in c2[int(ob["cbeg"]):int(ob["cend"])]:
+            for buf, off in ((c["a_buf"], c["a_off"]), (c["b_buf"], c["b_off"])):
+                if buf == 2 and int(off) not in done:
+                    j = int(np.searchsorted(mid_start, off))
+                    acc, _ = _window(bufs, c1, o1[j])
+                    _store(mid, o1[j], acc)
+                    done.add(int(off))
+        out[int(w)], _ = _window(bufs, c2, ob)
+    return out

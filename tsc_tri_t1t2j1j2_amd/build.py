@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtsc_hotpath.so")
-SOURCES = ["tsc_capi.hip", "tsc_grouped.hip", "tsc_pk.hip", "tsc_gd.hip", "tsc_vec.hip", "tsc_rho.hip", "tsc_planner.hip"]
+SOURCES = ["tsc_capi.hip", "tsc_grouped.hip", "tsc_pk.hip", "tsc_gd.hip", "tsc_vec.hip", "tsc_rho.hip", "tsc_planner.hip", "tsc_loop.hip"]
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIPFLAGS = [

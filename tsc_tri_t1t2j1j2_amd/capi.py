@@ -40,10 +40,10 @@ EXPORTS = [
     "tsc_vec_norm", "tsc_vec_dptr", "tsc_vec_fill_gaussian", "tsc_vec_axpby", "tsc_vec_copy",
     "tsc_vec_copy_range", "tsc_vec_upload_range", "tsc_vec_download_range",
     "tsc_plan_create", "tsc_plan_create_dev", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
-    "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_graph", "tsc_plan_last_times",
+    "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_loop", "tsc_hpsi_graph", "tsc_plan_last_times",
     "tsc_plan_autotune",
     "tsc_lanczos", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_select_states",
-    "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_copy",
+    "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_flops", "tsc_tasks_copy",
     "tsc_tasks_plan", "tsc_tasks_plan_dev", "tsc_tasks_destroy",
 ]
 
@@ -104,6 +104,7 @@ def lib():
         "tsc_hpsi": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
         "tsc_hpsi_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
         "tsc_hpsi_graph": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+        "tsc_hpsi_loop": (c_int, [c_void_p, c_void_p, c_void_p]),
         "tsc_plan_last_times": (c_int, [c_void_p, P(c_float), P(c_float)]),
         "tsc_plan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, P(c_double)]),
         "tsc_lanczos": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_int, c_int, P(c_double), P(c_int32)]),
@@ -318,6 +319,11 @@ class Plan:
 
     def apply(self, vin, vout, prec=PREC_F64):
         check(lib().tsc_hpsi(self.h, vin.h, vout.h, int(prec)))
+
+    def apply_loop(self, vin, vout):
+        """FP64 H.psi as one ZGEMM call per contribution (the per-product
+        baseline; plans of GEMM contributions only)."""
+        check(lib().tsc_hpsi_loop(self.h, vin.h, vout.h))
 
     def apply_graph(self, vin, vout, prec=PREC_F64):
         check(lib().tsc_hpsi_graph(self.h, vin.h, vout.h, int(prec)))

@@ -17,6 +17,7 @@
 // conj(<j||c+||j'>), fermion sign (-1)^N of the part an odd operator passes.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <array>
@@ -289,6 +290,9 @@ struct tsc_tasks {
   int64_t arena_len = 0;           // operator arena length the descriptors address
   std::vector<int32_t> super_xy;   // (x, y, dx, dy) per psi sector
   std::vector<int32_t> lsec, rsec; // (n, 2J, dim) of the enlarged bases
+  // algorithmic flops with every (L transition, R transition, source
+  // window) product charged at its cheaper order and nothing shared
+  double flops_pairmin = 0.0;
 };
 
 namespace {
@@ -303,7 +307,8 @@ struct MidKey {
 struct Builder {
   tsc_tasks *T;
   std::vector<std::vector<tsc_contrib>> per_out;
-  std::map<MidKey, int64_t> mid;   // (L matrix, psi sector, row pos, col pos) -> MID offset
+  std::map<MidKey, int64_t> mid;    // (L matrix, psi sector, row pos, col pos) -> MID offset of O_L psi
+  std::map<MidKey, int64_t> mid_r;  // (R matrix, psi sector, row pos, col pos) -> MID offset of psi O_R^T
 
   static tsc_contrib make(int64_t aoff, int lda, int abuf, int aop, int64_t boff, int ldb, int bbuf, int bop, int k,
                           double f) {
@@ -392,15 +397,33 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
   bd.T = T;
   bd.per_out.resize(T->o2.size());
 
+  // contraction order of the two-sided products (both sides a stored
+  // matrix): "left" forms O_L psi first, "right" psi O_R^T first, "auto"
+  // (default) takes per source window and term whichever costs fewer
+  // flops, intermediates shared between the products of the window (and
+  // ones already formed by earlier terms) counted once. TSC_PLAN_ORDER
+  // overrides it (tests compare the orders).
+  int order_mode = 0;
+  if (const char *e = std::getenv("TSC_PLAN_ORDER")) {
+    if (std::strcmp(e, "left") == 0) order_mode = 1;
+    else if (std::strcmp(e, "right") == 0) order_mode = 2;
+  }
+  struct PairRec {
+    const Tr *tl, *tr;
+    double f;
+    int w;
+  };
   auto cross = [&](const std::vector<Tr> &TL, const std::vector<Tr> &TR, double g, bool cut, int k2, bool y_odd) {
     std::vector<std::vector<const Tr *>> byx(L.sec.size()), byy(R.sec.size());
     for (const Tr &t : TL) byx[t.x].push_back(&t);
     for (const Tr &t : TR) byy[t.x].push_back(&t);
+    std::vector<PairRec> both;
     for (int i = 0; i < (int)sb.size(); ++i) {
       const int x = sb[i][0], y = sb[i][1], dx = sb[i][2];
       if (byx[x].empty() || byy[y].empty()) continue;
       const int Jx = L.sec[x].j, Jy = R.sec[y].j;
       const double fsgn = (y_odd && (L.sec[x].n % 2)) ? -1.0 : 1.0;
+      both.clear();
       for (const Tr *tl : byx[x])
         for (const Tr *tr : byy[y]) {
           auto it = sbi.find({tl->x2, tr->x2});
@@ -412,9 +435,58 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
           const int w = win[wkey(i2, tl->p2, tr->p2)];
           const int64_t src = soff[i] + tl->p + (int64_t)tr->p * dx;
           if (tl->mat >= 0 && tr->mat >= 0) {
+            both.push_back(PairRec{tl, tr, f, w});
+          } else if (tl->mat >= 0) {
             const MatRef &lm = cx.mats[tl->mat];
+            bd.per_out[w].push_back(Builder::make(lm.off, lm.ld, TSC_BUF_OPS, lm.op, src, std::max(1, dx), TSC_BUF_IN,
+                                                  TSC_OP_N, tl->d, f));
+            T->flops_pairmin += 8.0 * tl->d2 * tl->d * tr->d;
+          } else if (tr->mat >= 0) {
             const MatRef &rm = cx.mats[tr->mat];
-            const MidKey key{tl->mat, i, tl->p, tr->p};
+            bd.per_out[w].push_back(Builder::make(src, std::max(1, dx), TSC_BUF_IN, TSC_OP_N, rm.off, rm.ld,
+                                                  TSC_BUF_OPS, rm.op == TSC_OP_N ? TSC_OP_T : TSC_OP_CONJ, tr->d, f));
+            T->flops_pairmin += 8.0 * tl->d * tr->d * tr->d2;
+          } else {
+            bd.per_out[w].push_back(Builder::make(src, std::max(1, dx), TSC_BUF_IN, TSC_OP_N, 0, 1, 0, TSC_OP_I, 0, f));
+            T->flops_pairmin += 8.0 * tl->d * tr->d;
+          }
+        }
+      // two-sided products, one group per source window (row pos, col pos)
+      std::stable_sort(both.begin(), both.end(), [](const PairRec &u, const PairRec &v) {
+        return u.tl->p != v.tl->p ? u.tl->p < v.tl->p : u.tr->p < v.tr->p;
+      });
+      for (size_t g0 = 0; g0 < both.size();) {
+        size_t g1 = g0;
+        while (g1 < both.size() && both[g1].tl->p == both[g0].tl->p && both[g1].tr->p == both[g0].tr->p) ++g1;
+        const int p = both[g0].tl->p, q = both[g0].tr->p;
+        const double dl = both[g0].tl->d, dr = both[g0].tr->d;
+        const int64_t src = soff[i] + p + (int64_t)q * dx;
+        double cl = 0.0, cr = 0.0;
+        std::vector<int> seen_l, seen_r;
+        for (size_t k = g0; k < g1; ++k) {
+          const Tr *tl = both[k].tl, *tr = both[k].tr;
+          const double d2l = tl->d2, d2r = tr->d2;
+          if (std::find(seen_l.begin(), seen_l.end(), tl->mat) == seen_l.end()) {
+            seen_l.push_back(tl->mat);
+            if (!bd.mid.count(MidKey{tl->mat, i, p, q})) cl += d2l * dl * dr;
+          }
+          if (std::find(seen_r.begin(), seen_r.end(), tr->mat) == seen_r.end()) {
+            seen_r.push_back(tr->mat);
+            if (!bd.mid_r.count(MidKey{tr->mat, i, p, q})) cr += dl * dr * d2r;
+          }
+          cl += d2l * dr * d2r;
+          cr += d2l * dl * d2r;
+          T->flops_pairmin += 8.0 * std::min(d2l * dl * dr + d2l * dr * d2r, dl * dr * d2r + d2l * dl * d2r);
+        }
+        const bool left = order_mode == 1 || (order_mode == 0 && cl <= cr);
+        for (size_t k = g0; k < g1; ++k) {
+          const Tr *tl = both[k].tl, *tr = both[k].tr;
+          const MatRef &lm = cx.mats[tl->mat];
+          const MatRef &rm = cx.mats[tr->mat];
+          const int rop = rm.op == TSC_OP_N ? TSC_OP_T : TSC_OP_CONJ;
+          if (left) {
+            // MID = O_L psi_window (d2l x dr); out += f MID op(O_R)^T
+            const MidKey key{tl->mat, i, p, q};
             int64_t moff;
             auto mit = bd.mid.find(key);
             if (mit != bd.mid.end()) {
@@ -436,20 +508,38 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
               T->c1.push_back(Builder::make(lm.off, lm.ld, TSC_BUF_OPS, lm.op, src, std::max(1, dx), TSC_BUF_IN,
                                             TSC_OP_N, tl->d, 1.0));
             }
-            bd.per_out[w].push_back(Builder::make(moff, std::max(1, tl->d2), TSC_BUF_MID, TSC_OP_N, rm.off, rm.ld,
-                                                  TSC_BUF_OPS, rm.op == TSC_OP_N ? TSC_OP_T : TSC_OP_CONJ, tr->d, f));
-          } else if (tl->mat >= 0) {
-            const MatRef &lm = cx.mats[tl->mat];
-            bd.per_out[w].push_back(Builder::make(lm.off, lm.ld, TSC_BUF_OPS, lm.op, src, std::max(1, dx), TSC_BUF_IN,
-                                                  TSC_OP_N, tl->d, f));
-          } else if (tr->mat >= 0) {
-            const MatRef &rm = cx.mats[tr->mat];
-            bd.per_out[w].push_back(Builder::make(src, std::max(1, dx), TSC_BUF_IN, TSC_OP_N, rm.off, rm.ld,
-                                                  TSC_BUF_OPS, rm.op == TSC_OP_N ? TSC_OP_T : TSC_OP_CONJ, tr->d, f));
+            bd.per_out[both[k].w].push_back(Builder::make(moff, std::max(1, tl->d2), TSC_BUF_MID, TSC_OP_N, rm.off,
+                                                          rm.ld, TSC_BUF_OPS, rop, tr->d, both[k].f));
           } else {
-            bd.per_out[w].push_back(Builder::make(src, std::max(1, dx), TSC_BUF_IN, TSC_OP_N, 0, 1, 0, TSC_OP_I, 0, f));
+            // MID = psi_window op(O_R)^T (dl x d2r); out += f O_L MID
+            const MidKey key{tr->mat, i, p, q};
+            int64_t moff;
+            auto mit = bd.mid_r.find(key);
+            if (mit != bd.mid_r.end()) {
+              moff = mit->second;
+            } else {
+              moff = T->mid_len;
+              T->mid_len += (int64_t)tl->d * tr->d2;
+              bd.mid_r.emplace(key, moff);
+              tsc_outblock ob;
+              std::memset(&ob, 0, sizeof(ob));
+              ob.c_off = moff;
+              ob.ldc = std::max(1, tl->d);
+              ob.m = tl->d;
+              ob.n = tr->d2;
+              ob.c_buf = TSC_BUF_MID;
+              ob.cbeg = (int32_t)T->c1.size();
+              ob.cend = ob.cbeg + 1;
+              T->o1.push_back(ob);
+              T->c1.push_back(Builder::make(src, std::max(1, dx), TSC_BUF_IN, TSC_OP_N, rm.off, rm.ld, TSC_BUF_OPS, rop,
+                                            tr->d, 1.0));
+            }
+            bd.per_out[both[k].w].push_back(Builder::make(lm.off, lm.ld, TSC_BUF_OPS, lm.op, moff, std::max(1, tl->d),
+                                                          TSC_BUF_MID, TSC_OP_N, tl->d, both[k].f));
           }
         }
+        g0 = g1;
+      }
     }
   };
 
@@ -501,6 +591,20 @@ extern "C" int tsc_tasks_info(const tsc_tasks *T, int64_t *counts) {
   counts[6] = (int64_t)T->super_xy.size() / 4;
   counts[7] = (int64_t)T->lsec.size() / 3;
   counts[8] = (int64_t)T->rsec.size() / 3;
+  return TSC_OK;
+}
+
+extern "C" int tsc_tasks_flops(const tsc_tasks *T, double *flops2) {
+  TSC_REQUIRE(T && flops2, "tsc_tasks_flops arguments");
+  double ex = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    const std::vector<tsc_contrib> &c = s == 0 ? T->c1 : T->c2;
+    for (const tsc_outblock &ob : (s == 0 ? T->o1 : T->o2))
+      for (int k = ob.cbeg; k < ob.cend; ++k)
+        ex += 8.0 * ob.m * ob.n * ((c[k].a_op == TSC_OP_I || c[k].b_op == TSC_OP_I) ? 1.0 : (double)c[k].k);
+  }
+  flops2[0] = ex;
+  flops2[1] = T->flops_pairmin;
   return TSC_OK;
 }
 

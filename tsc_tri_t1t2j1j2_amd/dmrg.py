@@ -70,7 +70,7 @@ class DMRG:
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
                  prec=capi.PREC_F64, refine=None, seed=1234, log=None, native_planner=True, resident=True,
                  lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True, guessed_f64=False,
-                 autotune_flops=None, retain=None):
+                 autotune_flops=None, retain=None, warmup_ramp=None, on_plan=None):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -114,6 +114,15 @@ class DMRG:
         # runs that stop at the centre, whose stored blocks would not fit in
         # HBM at large M
         self.retain = retain
+        # warmup_ramp = (m0, n): the warm-up keeps at most m0 multiplets except
+        # in its last n steps, which keep kept_max (reaches the centre blocks
+        # of a large kept_max at a fraction of the cost; None: kept_max
+        # throughout, as the reference's warm-up)
+        self.warmup_ramp = None if warmup_ramp is None else (int(warmup_ramp[0]), int(warmup_ramp[1]))
+        # on_plan(dmrg, sb, plan, psi, tasks, step): called in a resident step
+        # after the superblock plan and the start vector exist, before the
+        # solve (measurement hook; must leave psi unchanged)
+        self.on_plan = on_plan
         self.sys_trun, self.env_trun = {}, {}   # block length -> wavetrans.Trunc that built it
         self.seed = seed
         self.native_planner = native_planner
@@ -391,10 +400,10 @@ class DMRG:
         for v in spec.values():
             step.add_merges(v[4])
         step.finalize()
+        tasks = None
         if self.native_planner:
             tasks = native.NativeTasks(sb, step=step, terms=terms)
             plan = tasks.plan(ctx)
-            tasks.close()
         else:
             plan = sb.plan_builder(resolver=step.offset, terms=terms).build_dev(ctx, step.vec, sb.super.dim,
                                                                                 sb.super.dim)
@@ -402,6 +411,12 @@ class DMRG:
         self.timers["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         psi = made["psi"] = self._initial(sb, guess)
+        if self.on_plan is not None:
+            th = time.perf_counter()
+            self.on_plan(self, sb, plan, psi, tasks, step)
+            t0 += time.perf_counter() - th   # the hook's time is not the solve's
+        if tasks is not None:
+            tasks.close()
         t0 += self._autotune(plan, psi)   # reported under "autotune", not "lanczos"
         psi0 = made["psi0"] = self._shadow_start(psi)
         # the truncation-independent half of the renormalisation plans is host
@@ -493,10 +508,13 @@ class DMRG:
             basis_dim_sys = sum(d for (_, _, d) in sys.sectors) * 2
             basis_dim_env = sum(d for (_, _, d) in env.sectors) * 2
             from .blocks import TotalBasis
-            ident_s = TotalBasis(sys.sectors).dim <= self.kept_max
-            ident_e = TotalBasis(env.sectors).dim <= self.kept_max
+            kmax = self.kept_max
+            if self.warmup_ramp is not None and label + 1 <= end_len - self.warmup_ramp[1]:
+                kmax = self.warmup_ramp[0]
+            ident_s = TotalBasis(sys.sectors).dim <= kmax
+            ident_e = TotalBasis(env.sectors).dim <= kmax
             del basis_dim_sys, basis_dim_env
-            r = self.step(sys, env, s1, s2, bonds, ntot, tol, "both", self.kept_max, self.trun_err * 1e2,
+            r = self.step(sys, env, s1, s2, bonds, ntot, tol, "both", kmax, self.trun_err * 1e2,
                           identity_sys=ident_s, identity_env=ident_e, partners=partners)
             sys, env = r["sys"], r["env"]
             self.sys_blocks[l + 1], self.env_blocks[l + 1] = sys, env
@@ -523,16 +541,23 @@ class DMRG:
                     arena.close()
                 truns.pop(L, None)
 
-    def centre_window(self, nsteps, tol, kept_max, trun_err, tag="window"):
+    def centre_window(self, nsteps, tol, kept_max, trun_err, tag="window", shadow_from=None):
         """After warm_up(): `nsteps` left-to-right sweep steps from the centre
         (l = N/2-1, ...), as the first half-sweep runs them (the first from
         a random start, the others from the wave-function guess). The blocks
         they grow are dropped afterwards, so the window can be run again from
         the same warm-up blocks in another precision. Returns one record per
         step: energy, wall seconds (without FP64 shadow solves), H.psi calls,
-        superblock dimension and sectors, kept states, discarded weight."""
+        superblock dimension and sectors, kept states, discarded weight.
+        Every call starts from the same seeded random vector. shadow_from =
+        k: the steps from the k-th (0-based) on are also solved in FP64 from
+        their start vectors (self.shadow); None leaves shadow_f64 as set."""
         N = self.N
         l0 = N // 2 - 1
+        if not hasattr(self, "_window_nstep"):
+            self._window_nstep = self.nstep
+        self.nstep = self._window_nstep
+        shadow_setting = self.shadow_f64
         if self.sys_blocks.get(l0) is None or any(self.env_blocks.get(N - l - 2) is None
                                                    for l in range(l0, l0 + nsteps)):
             raise ValueError("the centre window needs the warm-up blocks of lengths N/2-1 and below")
@@ -541,6 +566,8 @@ class DMRG:
         prev = None
         try:
             for l in range(l0, l0 + nsteps):
+                if shadow_from is not None:
+                    self.shadow_f64 = l - l0 >= shadow_from
                 guess = self._guess("LR", prev, self.sys_trun.get(l), self.env_trun.get(N - l - 1))
                 sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
                 r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err,
@@ -559,6 +586,7 @@ class DMRG:
                 self._say(f"{tag} l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
                           f"dim={r['dim']} nsec={r['nsec']} mv={r['matvecs']} {r['seconds']:.2f}s")
         finally:
+            self.shadow_f64 = shadow_setting
             if prev is not None:
                 prev[0].close()
             for L, (blk, tr) in saved.items():

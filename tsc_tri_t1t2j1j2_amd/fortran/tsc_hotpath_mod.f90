@@ -204,6 +204,10 @@ module tsc_hotpath
       type(c_ptr), value :: plan, vin, vout
       integer(c_int), value :: prec
     end function
+    integer(c_int) function tsc_hpsi_loop(plan, vin, vout) bind(c, name="tsc_hpsi_loop")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan, vin, vout
+    end function
     integer(c_int) function tsc_hpsi_graph(plan, vin, vout, prec) bind(c, name="tsc_hpsi_graph")
       import :: c_int, c_ptr
       type(c_ptr), value :: plan, vin, vout
@@ -235,6 +239,10 @@ module tsc_hotpath
       integer(c_int32_t), value :: nsec, kept_min, kept_max
       real(c_double), value :: trun_err
       integer(c_int64_t), value :: u_cap
+    end function
+    integer(c_int) function tsc_tasks_flops(tasks, flops2) bind(c, name="tsc_tasks_flops")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tasks, flops2
     end function
     integer(c_int) function tsc_superblock_tasks(desc, tasks) bind(c, name="tsc_superblock_tasks")
       import :: c_int, c_ptr

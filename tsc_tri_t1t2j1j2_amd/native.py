@@ -54,6 +54,8 @@ def _bind():
     L.tsc_superblock_tasks.restype = ctypes.c_int
     L.tsc_tasks_info.argtypes = [ctypes.c_void_p, P(c_int64)]
     L.tsc_tasks_info.restype = ctypes.c_int
+    L.tsc_tasks_flops.argtypes = [ctypes.c_void_p, P(c_double)]
+    L.tsc_tasks_flops.restype = ctypes.c_int
     L.tsc_tasks_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 7
     L.tsc_tasks_copy.restype = ctypes.c_int
     L.tsc_tasks_plan.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, c_int64, P(ctypes.c_void_p)]
@@ -134,6 +136,12 @@ class NativeTasks:
         capi.check(L.tsc_tasks_info(h, cnt))
         self.counts = list(cnt)
         self.mid_len, self.psi_len = cnt[4], cnt[5]
+
+    def flops(self):
+        """(flops as planned, per-product minimum without sharing)."""
+        out = (c_double * 2)()
+        capi.check(_bind().tsc_tasks_flops(self.h, out))
+        return out[0], out[1]
 
     def arrays(self):
         L = _bind()

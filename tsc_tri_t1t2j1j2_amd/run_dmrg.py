@@ -56,6 +56,12 @@ def main(argv=None):
     ap.add_argument("--window-precs", default="bf16x3,f64",
                     help="precisions of the centre window; a mixed one is also solved in FP64 from the same "
                          "start vectors (same-path energy error)")
+    ap.add_argument("--window-lanzero", type=float, default=None,
+                    help="Lanczos tolerance base of the centre window (default: --lanzero); the window solves to "
+                         "15 times it, as the first half-sweep does")
+    ap.add_argument("--warmup-ramp", default=None, metavar="M0,N",
+                    help="warm-up keeping at most M0 multiplets except in its last N steps (which keep "
+                         "--kept-max)")
     ap.add_argument("--dump-centre", default=None, metavar="FILE",
                     help="with --window: write the sector lists (N, 2S, multiplets) of the two centre blocks "
                          "after the warm-up, and the window records, to FILE (JSON)")
@@ -130,7 +136,8 @@ def centre_window(ctx, a):
     p0 = capi.PREC_NAMES[precs[0]]
     d = DMRG(ctx, a.Lx, a.Ny, t1=a.t1, t2=a.t2, J1=a.J1, J2=a.J2, delta=a.delta, kept_min=a.kept_min,
              kept_max=a.kept_max, sweeps=0, lanzero=a.lanzero, trun_err=a.trun_err, prec=p0,
-             refine=a.refine if p0 != capi.PREC_F64 else 0, seed=a.seed, log=log, retain=a.window + 1)
+             refine=a.refine if p0 != capi.PREC_F64 else 0, seed=a.seed, log=log, retain=a.window + 1,
+             warmup_ramp=tuple(int(x) for x in a.warmup_ramp.split(",")) if a.warmup_ramp else None)
     d.warm_up()
     t_warm = time.perf_counter() - t0
     l0 = d.N // 2 - 1
@@ -144,8 +151,9 @@ def centre_window(ctx, a):
     if a.dump_centre:
         with open(a.dump_centre, "w") as fh:
             json.dump(out, fh)
-    lanerr = a.lanzero
+    lanerr = a.lanzero if a.window_lanzero is None else a.window_lanzero
     tol, terr = lanerr * 15, a.trun_err * 15   # the first half-sweep's tolerances (main.f90:124-131)
+    out["window_tol"] = tol
     for name in precs:
         p = capi.PREC_NAMES[name]
         d.prec, d.refine = p, (a.refine if p != capi.PREC_F64 else 0)

@@ -1,18 +1,24 @@
-"""Synthetic DMRG superblocks with the structure of a real step.
+"""DMRG superblocks with the structure of a real step and random values.
 
 A superblock of the YC cylinder at its centre: the sys block holds sites
 1..l, the env block sites l+3..N, and the centre sites are l+1, l+2. The
-block sectors (N, 2S) and their multiplet dimensions follow a Gaussian
-profile around the mean filling, normalised to M kept multiplets; every
-operator block allowed by the quantum numbers is filled with seeded random
-numbers, so the superblock has exactly the sector structure, term list and
-GEMM shapes of a DMRG step at that M, with values that carry no physics.
+block sectors (N, 2S) and their multiplet dimensions are those of the
+centre blocks of a real DMRG run (real_centre_sectors: the YC6 6x24 blocks
+at M=6000, stored in data/yc6_24x6_m6000_centre.json, optionally with every
+sector's multiplets scaled), or, for tests of small sizes, a Gaussian
+profile around the mean filling (sector_profile). Every operator block
+allowed by the quantum numbers is filled with seeded random numbers, so the
+superblock has the sector structure, term list and GEMM shapes of a DMRG
+step, with values that carry no physics.
 
 Hermiticity is kept: block Hamiltonians are Hermitian per sector, n is
 Hermitian, and the S blocks obey the adjoint relation of a Hermitian rank-1
 tensor, <j'||S||j> = (-1)^(j'-j) conj(<j||S||j'>), so H is Hermitian and
 Lanczos is meaningful (used to compare mixed precision against FP64).
 """
+import json
+import os
+
 import numpy as np
 
 from . import lattice
@@ -77,8 +83,25 @@ def random_block(sites, keep, sectors, seed):
     return TotalModel(sites, sectors, H, ops)
 
 
-def centre_superblock(Lx, Ny, M, t1=1.0, t2=0.2, J1=0.4, J2=None, delta=12, seed=7):
-    """(sys, env, s1, s2, bonds, Ntot) for the centre step at M multiplets."""
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def real_centre_sectors(scale=1.0):
+    """(sys sectors, env sectors, description) of the YC6 6x24 centre blocks
+    at M=6000 from a real run, every sector's multiplets times `scale`
+    (rounded, at least 1)."""
+    with open(os.path.join(DATA, "yc6_24x6_m6000_centre.json")) as fh:
+        d = json.load(fh)
+
+    def sc(secs):
+        return [(int(n), int(s), max(1, int(round(m * scale)))) for (n, s, m) in secs]
+
+    return sc(d["sys_sectors"]), sc(d["env_sectors"]), d
+
+
+def centre_superblock(Lx, Ny, M, t1=1.0, t2=0.2, J1=0.4, J2=None, delta=12, seed=7, sectors=None):
+    """(sys, env, s1, s2, bonds, Ntot) for the centre step at M multiplets;
+    sectors = (sys sectors, env sectors) replaces the Gaussian profile."""
     N = Lx * Ny
     J2 = J1 * (t2 / t1) ** 2 if J2 is None else J2
     bonds = lattice.bonds(Lx, Ny, t1, t2, J1, J2)
@@ -93,8 +116,11 @@ def centre_superblock(Lx, Ny, M, t1=1.0, t2=0.2, J1=0.4, J2=None, delta=12, seed
         nb.setdefault(b, []).append(a)
     keep_sys = [i for i in sys_sites if any(j > l for j in nb.get(i, []))]
     keep_env = [i for i in env_sites if any(j < l + 3 for j in nb.get(i, []))]
-    n0 = int(round(l * ntot / N))
-    secs = sector_profile(M, n0)
-    sys = random_block(sys_sites, keep_sys, secs, seed)
-    env = random_block(env_sites, keep_env, secs, seed + 1)
+    if sectors is None:
+        n0 = int(round(l * ntot / N))
+        sys_secs = env_secs = sector_profile(M, n0)
+    else:
+        sys_secs, env_secs = sectors
+    sys = random_block(sys_sites, keep_sys, sys_secs, seed)
+    env = random_block(env_sites, keep_env, env_secs, seed + 1)
     return sys, env, s1, s2, bonds, ntot
