@@ -2,7 +2,7 @@
 
 Default workload: the YC6 6x24 t1-t2-J1-J2 cylinder (144 sites, 132
 electrons, t2 = 0.3, J2 = 0.036) at M = 6000 multiplets. A DMRG warm-up
-grows the blocks to the centre (kept_max 2000, its last three steps 6000),
+grows the blocks to the centre (kept_max 2000, its last six steps 6000),
 then the first half-sweep's centre steps run twice from the same blocks:
 mixed precision (BF16x3 H.psi, FP64 Lanczos and refinement) and FP64. At
 the first centre step, before its solve, the H.psi of that real superblock
@@ -51,7 +51,9 @@ def parse():
                          "sector histogram (x2 per sector for M=12000) with seeded random block values, H.psi only")
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "f32", "f64"])
     ap.add_argument("--window-steps", type=int, default=3, help="centre steps timed per precision")
-    ap.add_argument("--ramp", default="2000,3", help="warm-up kept_max and the number of final steps at M")
+    ap.add_argument("--ramp", default="2000,6", help="warm-up kept_max and the number of final steps at M "
+                                                      "(at least the window steps + 3, so that every block the "
+                                                      "window reads was grown from an M-block)")
     ap.add_argument("--warmup-lanzero", type=float, default=1e-6)
     ap.add_argument("--window-lanzero", type=float, default=1e-8)
     ap.add_argument("--refine", type=int, default=2, help="steps of the refinement pass after each mixed solve "

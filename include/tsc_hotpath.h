@@ -224,6 +224,14 @@ int tsc_plan_autotune(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precisio
  */
 int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer,
                 int precision, int refine_iters, double *energy, int32_t *info);
+/*
+ * H.psi products of the last tsc_lanczos call on p that ran as replays of a
+ * captured hipGraph. Plans below 2e11 flops per H.psi replay every product
+ * (the graph is captured once per precision on two vectors of the plan, the
+ * Krylov vectors are copied in and out); larger plans launch eagerly.
+ * Diagnostic; has no counterpart in the reference.
+ */
+int tsc_lanczos_replays(const tsc_plan *p, int32_t *replays);
 
 /*
  * ---- density matrix, eigensolve, truncation (den_mat.f90) ---------------

@@ -309,6 +309,32 @@ def test_lanczos_matches_exact_and_oracle(ctx, prec):
     assert np.linalg.norm(H @ psi - e * psi) < 1e-4
 
 
+@pytest.mark.parametrize("prec", [capi.PREC_F64, capi.PREC_BF16X3])
+def test_lanczos_graph_replays_equal_eager_launches(ctx, prec, monkeypatch):
+    """A small plan's Lanczos runs every H.psi as a replay of its captured
+    graph (with the loop's eager vector work and host synchronisations
+    between replays); the energy and Ritz vector equal those of eager
+    launches, on a first call (capture) and on a second one (reuse)."""
+    hs = HermitianSuperblock(dL=[40, 72, 24, 96], dR=[56, 32, 48, 64], npairs=6, seed=8)
+    plan = hs.plan(ctx)
+    rng = np.random.default_rng(9)
+    psi0 = rng.normal(size=hs.n) + 1j * rng.normal(size=hs.n)
+    runs = []
+    for eager in (False, False, True):
+        if eager:
+            monkeypatch.setenv("TSC_LANCZOS_EAGER", "1")
+        v = ctx.vec_from(psi0)
+        e, (nmv, _) = plan.lanczos(v, tol=1e-12, max_inner=60, max_outer=2, prec=prec,
+                                   refine_iters=4 if prec != capi.PREC_F64 else 0)
+        runs.append((e, v.download(), nmv, plan.lanczos_replays()))
+        v.close()
+    (e1, x1, n1, r1), (e2, x2, n2, r2), (e3, x3, n3, r3) = runs
+    assert r1 == n1 and r2 == n2 and r3 == 0
+    assert n1 == n2 == n3
+    assert abs(e1 - e3) < 1e-12 and abs(e2 - e3) < 1e-12
+    assert np.linalg.norm(x1 - x3) < 1e-10 and np.linalg.norm(x2 - x3) < 1e-10
+
+
 def _sectors(rng, dims):
     secs = [rng.normal(size=(d, e)) + 1j * rng.normal(size=(d, e)) for d, e in dims]
     nrm = np.sqrt(sum(np.sum(np.abs(x) ** 2) for x in secs))

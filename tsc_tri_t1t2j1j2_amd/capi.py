@@ -42,7 +42,7 @@ EXPORTS = [
     "tsc_plan_create", "tsc_plan_create_dev", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
     "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_loop", "tsc_hpsi_graph", "tsc_plan_last_times",
     "tsc_plan_autotune",
-    "tsc_lanczos", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_select_states",
+    "tsc_lanczos", "tsc_lanczos_replays", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_select_states",
     "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_flops", "tsc_tasks_copy",
     "tsc_tasks_plan", "tsc_tasks_plan_dev", "tsc_tasks_destroy",
 ]
@@ -108,6 +108,7 @@ def lib():
         "tsc_plan_last_times": (c_int, [c_void_p, P(c_float), P(c_float)]),
         "tsc_plan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, P(c_double)]),
         "tsc_lanczos": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_int, c_int, P(c_double), P(c_int32)]),
+        "tsc_lanczos_replays": (c_int, [c_void_p, P(c_int32)]),
         "tsc_density_matrix": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int, c_void_p]),
         "tsc_truncation_operator": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
                                             c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
@@ -347,6 +348,12 @@ class Plan:
         r = c_double(0.0 if target is None else float(target))
         flags = (1 if force else 0) | (2 if target is not None else 0)
         check(lib().tsc_plan_autotune(self.h, vin.h, vout.h, int(prec), flags, ctypes.byref(r)))
+        return r.value
+
+    def lanczos_replays(self):
+        """H.psi products of the last lanczos() that ran as graph replays."""
+        r = c_int32()
+        check(lib().tsc_lanczos_replays(self.h, ctypes.byref(r)))
         return r.value
 
     def lanczos(self, psi, tol=1e-10, max_inner=100, max_outer=4, prec=PREC_F64, refine_iters=0):

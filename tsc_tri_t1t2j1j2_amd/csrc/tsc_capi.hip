@@ -2,6 +2,7 @@
 // and the Lanczos driver.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -402,9 +403,12 @@ extern "C" int tsc_plan_destroy(tsc_plan *p) {
   tsc::pk_free(p->pk);
   tsc::gd_free(p->gd);
   for (auto v : p->kry) hipFree(v);
+  hipFree(p->lz_io[0]);
+  hipFree(p->lz_io[1]);
   for (int i = 0; i < 3; ++i) {
     if (p->ev[i]) hipEventDestroy(p->ev[i]);
     if (p->gexec[i]) hipGraphExecDestroy(p->gexec[i]);
+    if (p->lz_gexec[i]) hipGraphExecDestroy(p->lz_gexec[i]);
   }
   delete p;
   return TSC_OK;
@@ -663,6 +667,8 @@ void drop_graph(tsc_plan *p, int prec) {
   if (p->gexec[prec]) hipGraphExecDestroy(p->gexec[prec]);
   p->gexec[prec] = nullptr;
   p->g_in[prec] = p->g_out[prec] = nullptr;
+  if (p->lz_gexec[prec]) hipGraphExecDestroy(p->lz_gexec[prec]);
+  p->lz_gexec[prec] = nullptr;
 }
 }  // namespace
 
@@ -763,6 +769,47 @@ struct LanczosResult {
   int restarts;
 };
 
+// Plans below this many flops per H.psi replay their products from a
+// captured graph inside the Lanczos loop: their launches, not their
+// kernels, set the time. Larger plans launch eagerly (the two vector
+// copies a replay needs would cost more than the launches they save).
+constexpr double kLanczosGraphFlops = 2.0e11;
+
+// out = H in for Lanczos: eager, or a replay of the graph captured on the
+// plan's two fixed vectors with `in` copied in and the result copied out
+int lanczos_hpsi(tsc_plan *p, const double2 *in, double2 *out, int prec, bool replay) {
+  if (!replay) return hpsi_dev(p, in, out, prec);
+  hipStream_t s = p->ctx->stream;
+  const int64_t n = p->kry_n;
+  if (!p->lz_io[0]) {
+    TSC_HIP_CHECK(hipMalloc((void **)&p->lz_io[0], sizeof(double2) * std::max<int64_t>(n, 1)));
+    TSC_HIP_CHECK(hipMalloc((void **)&p->lz_io[1], sizeof(double2) * std::max<int64_t>(n, 1)));
+  }
+  if (!p->lz_gexec[prec]) {
+    int rc = ensure_precision(p, prec);
+    if (rc) return rc;
+    TSC_HIP_CHECK(hipStreamSynchronize(s));
+    hipGraph_t g;
+    TSC_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_hpsi(p, p->lz_io[0], p->lz_io[1], prec, false);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+      return TSC_EHIP;
+    }
+    e = hipGraphInstantiate(&p->lz_gexec[prec], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    TSC_HIP_CHECK(e);
+  }
+  TSC_HIP_CHECK(hipMemcpyAsync(p->lz_io[0], in, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+  TSC_HIP_CHECK(hipGraphLaunch(p->lz_gexec[prec], s));
+  TSC_HIP_CHECK(hipMemcpyAsync(out, p->lz_io[1], sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+  p->timed = false;
+  ++p->lz_replays;
+  return TSC_OK;
+}
+
 // prec: the first H.psi of each restart (from the start or Ritz vector);
 // prec_rest: the products of the later Krylov vectors
 int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_outer, int prec, int prec_rest,
@@ -776,8 +823,16 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
     hipStreamSynchronize(ctx->stream);
     for (auto v : p->kry) hipFree(v);
     p->kry.clear();
+    hipFree(p->lz_io[0]);
+    hipFree(p->lz_io[1]);
+    p->lz_io[0] = p->lz_io[1] = nullptr;
+    for (int i = 0; i < 3; ++i) {
+      if (p->lz_gexec[i]) hipGraphExecDestroy(p->lz_gexec[i]);
+      p->lz_gexec[i] = nullptr;
+    }
     p->kry_n = n;
   }
+  const bool replay = p->flops < kLanczosGraphFlops && std::getenv("TSC_LANCZOS_EAGER") == nullptr;
   while ((int)p->kry.size() < max_inner + 1) {
     double2 *v = nullptr;
     TSC_HIP_CHECK(hipMalloc((void **)&v, sizeof(double2) * std::max<int64_t>(n, 1)));
@@ -818,7 +873,7 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
       LZ_CALL(dots_re(ctx, vv, 1, V[0], n, &nn));
       uu[0] = nn;
     }
-    LZ_CALL(hpsi_dev(p, V[0], V[1], prec));
+    LZ_CALL(lanczos_hpsi(p, V[0], V[1], prec, replay));
     ++matvecs;
     {
       const double2 *vv[1] = {V[0]};
@@ -861,7 +916,7 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
         uu[last] = nn;
       }
       ++last;
-      LZ_CALL(hpsi_dev(p, V[last - 1], V[last], prec_rest));
+      LZ_CALL(lanczos_hpsi(p, V[last - 1], V[last], prec_rest, replay));
       ++matvecs;
       {
         const double2 *vv[1] = {V[last - 1]};
@@ -907,6 +962,7 @@ extern "C" int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner,
   TSC_REQUIRE(p && psi && energy, "tsc_lanczos arguments");
   TSC_REQUIRE(max_outer >= 0 && refine_iters >= 0, "tsc_lanczos iteration counts");
   LanczosResult r{0.0, 0, 0};
+  p->lz_replays = 0;
   int rc = lanczos_pass(p, psi, tol, max_inner, max_outer, prec, prec, &r);
   if (rc) return rc;
   int mv = r.matvecs, rs = r.restarts;
@@ -928,5 +984,11 @@ extern "C" int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner,
     info[0] = mv;
     info[1] = rs;
   }
+  return TSC_OK;
+}
+
+extern "C" int tsc_lanczos_replays(const tsc_plan *p, int32_t *replays) {
+  TSC_REQUIRE(p && replays, "tsc_lanczos_replays arguments");
+  *replays = p->lz_replays;
   return TSC_OK;
 }

@@ -223,6 +223,11 @@ struct tsc_plan {
   // Lanczos Krylov vectors (kry_n elements each), kept between calls
   std::vector<double2 *> kry;
   int64_t kry_n = 0;
+  // Lanczos H.psi replays: graphs captured on two fixed vectors (lz_io,
+  // kry_n elements each) per precision; Krylov vectors are copied in and out
+  double2 *lz_io[2] = {nullptr, nullptr};
+  hipGraphExec_t lz_gexec[3] = {nullptr, nullptr, nullptr};
+  int lz_replays = 0;  // H.psi products of the last tsc_lanczos call that ran as graph replays
 };
 
 namespace tsc {

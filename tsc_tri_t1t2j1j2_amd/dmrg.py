@@ -293,15 +293,17 @@ class DMRG:
         the grown block(s) with the truncations that built them, and the
         ground state (device vector) with its superblock bases. guess(sb):
         Lanczos start vector for superblock sb, or None. out["seconds"] is
-        the step's wall time without the FP64 shadow solve."""
-        t_step, sh0 = time.perf_counter(), self.timers.get("shadow", 0.0)
+        the step's wall time without the FP64 shadow solve, the work-unit
+        tuning and the on_plan hook (each kept in its own timer)."""
+        aside = ("shadow", "autotune", "hook")
+        t_step, sh0 = time.perf_counter(), sum(self.timers.get(k, 0.0) for k in aside)
         if self.resident:
             out = self._step_resident(sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err,
                                       identity_sys, identity_env, partners, guess)
         else:
             out = self._step_host(sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys,
                                   identity_env, partners, guess)
-        out["seconds"] = time.perf_counter() - t_step - (self.timers.get("shadow", 0.0) - sh0)
+        out["seconds"] = time.perf_counter() - t_step - (sum(self.timers.get(k, 0.0) for k in aside) - sh0)
         return out
 
     def _step_host(self, sys, env, s1, s2, bonds, ntot, tol, grow, kept_max, trun_err, identity_sys,
@@ -414,7 +416,9 @@ class DMRG:
         if self.on_plan is not None:
             th = time.perf_counter()
             self.on_plan(self, sb, plan, psi, tasks, step)
-            t0 += time.perf_counter() - th   # the hook's time is not the solve's
+            dt = time.perf_counter() - th
+            t0 += dt   # the hook's time is not the solve's
+            self.timers["hook"] = self.timers.get("hook", 0.0) + dt
         if tasks is not None:
             tasks.close()
         t0 += self._autotune(plan, psi)   # reported under "autotune", not "lanczos"
@@ -568,6 +572,7 @@ class DMRG:
             for l in range(l0, l0 + nsteps):
                 if shadow_from is not None:
                     self.shadow_f64 = l - l0 >= shadow_from
+                t_before = dict(self.timers)
                 guess = self._guess("LR", prev, self.sys_trun.get(l), self.env_trun.get(N - l - 1))
                 sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
                 r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err,
@@ -580,8 +585,11 @@ class DMRG:
                 if prev is not None:
                     prev[0].close()
                 prev = (r["psi"], r["bases"])
+                parts = {k: round(v - t_before.get(k, 0.0), 3) for k, v in self.timers.items()
+                         if v - t_before.get(k, 0.0) > 0.0}
                 recs.append({"l": l, "E": r["E"], "seconds": r["seconds"], "matvecs": r["matvecs"],
-                             "dim": r["dim"], "nsec": r["nsec"], "kept": r["sys_kept"], "trun_err": r["sys_err"]})
+                             "dim": r["dim"], "nsec": r["nsec"], "kept": r["sys_kept"], "trun_err": r["sys_err"],
+                             "parts_s": parts})
                 self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
                 self._say(f"{tag} l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
                           f"dim={r['dim']} nsec={r['nsec']} mv={r['matvecs']} {r['seconds']:.2f}s")

@@ -224,6 +224,10 @@ module tsc_hotpath
       integer(c_int), value :: prec, flags
       real(c_double) :: chosen
     end function
+    integer(c_int) function tsc_lanczos_replays(plan, replays) bind(c, name="tsc_lanczos_replays")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan, replays
+    end function
     integer(c_int) function tsc_density_matrix(ctx, psi, nsec, sys_dim, env_dim, side, rho) &
         bind(c, name="tsc_density_matrix")
       import :: c_int, c_ptr, c_int32_t
