@@ -385,6 +385,81 @@ int tsc_tasks_plan(tsc_ctx *ctx, const tsc_tasks *t, const double *arena_c128, i
 int tsc_tasks_plan_dev(tsc_ctx *ctx, const tsc_tasks *t, const tsc_vec *arena, tsc_plan **plan);
 int tsc_tasks_destroy(tsc_tasks *t);
 
+/*
+ * ---- block models: model_wave / hmerge and Get_Hamiltonian_Sys/Env -------
+ * A block model is what the reference keeps in Total_Model
+ * (common.f90:84-174): the block's sectors, its Hamiltonian and, for every
+ * site with bonds leaving the block, that site's c+, S and n on the block
+ * (reduced elements, bra x ket column-major in one host arena). Bonds are
+ * the lattice's (i, j, t, J): hopping t, exchange J (the reference's
+ * jt / jd / jn, dmrg_sub.f90:453-540).
+ */
+typedef struct tsc_site_ops {
+  int32_t site;
+  const tsc_op_desc *c, *S, *n; /* up_dif/k2: c+ 1/1, S 0/2, n 0/0 */
+} tsc_site_ops;
+
+typedef struct tsc_model_desc {
+  int32_t nsec;
+  const tsc_sector *sec;
+  int32_t nsites;
+  const int32_t *sites; /* lattice sites of the block */
+  const tsc_op_desc *ham;
+  int32_t nkeep;
+  const tsc_site_ops *keep;
+} tsc_model_desc;
+
+typedef struct tsc_bond {
+  int32_t i, j;
+  double t, J;
+} tsc_bond;
+
+/*
+ * model_wave's task arrays for the superblock [sys, s1 | env, s2]
+ * (get_sys.f90:6915-7393) with hmerge's merging of partner operators
+ * (get_sys.f90:7398-7626): the bonds are classified by the parts they join,
+ * the operators of the partner sites of each class are summed with their
+ * couplings into merged operators (appended to a copy of the arena held by
+ * the tasks), and the planner above builds the contraction. Env-block
+ * operators act as psi . O^T (the conjugates of the reference's, which
+ * apply psi . O^H, get_sys.f90:1869). tsc_tasks_plan(ctx, t, NULL, 0, &p)
+ * uploads the tasks' own arena.
+ */
+int tsc_model_tasks(const tsc_model_desc *sys, const tsc_model_desc *env, int32_t s1, int32_t s2,
+                    int32_t nbonds, const tsc_bond *bonds, int32_t ntot, int32_t s2tot,
+                    const double *arena_c128, int64_t arena_len, tsc_tasks **out);
+
+/*
+ * Get_Hamiltonian_Sys / _Env followed by update_trun_dia / _ndia
+ * (get_sys.f90:4-592, block.f90:3465-3683) as one contraction plan: the
+ * enlarged block blk (x) new_site (sectors in Get_basis order,
+ * block.f90:2087-2189) with its Hamiltonian (block H + the merged
+ * block-site bonds) and the c+, S, n of every site of keep_sites, each
+ * truncated as U_x2^H O U_x. The truncation keeps the enlarged sectors
+ * order[0..norder) (the new block's sector order) with kept[q] states;
+ * the plan's input is U_x (dim_x x kept columns, column-major) laid end to
+ * end in that order, its output the new block's operators. Their layout
+ * comes from tsc_tasks_layout.
+ */
+int tsc_renorm_tasks(const tsc_model_desc *blk, int32_t new_site, int32_t nbonds, const tsc_bond *bonds,
+                     int32_t nkeep, const int32_t *keep_sites, int32_t norder, const int32_t *order,
+                     const int32_t *kept, const double *arena_c128, int64_t arena_len, tsc_tasks **out);
+
+/*
+ * Output layout of renormalisation tasks: counts3 = (windows, new sectors,
+ * enlarged sectors); layout[3w..3w+2] = (operator, bra, ket) of window w
+ * at output offset offsets[w] (operator -1: Hamiltonian; 3k + 0/1/2: c+, S,
+ * n of keep_sites[k]; bra/ket: new sector indices); new_sec = (N, 2S,
+ * kept) per new sector; enl_sec = (N, 2S, dim) per enlarged sector. Any
+ * output pointer may be NULL.
+ */
+int tsc_tasks_layout(const tsc_tasks *t, int64_t *counts3, int32_t *layout, int64_t *offsets,
+                     int32_t *new_sec, int32_t *enl_sec);
+/* lens3 = (input length, output length, length of the tasks' own arena) */
+int tsc_tasks_lengths(const tsc_tasks *t, int64_t *lens3);
+/* the tasks' own arena (lens3[2] complex128 values) */
+int tsc_tasks_copy_arena(const tsc_tasks *t, double *arena_c128);
+
 /* Racah 6j symbol with doubled arguments (W6JS, block.f90:3744-3795) */
 double tsc_w6j(int32_t j1, int32_t j2, int32_t j3, int32_t l1, int32_t l2, int32_t l3);
 

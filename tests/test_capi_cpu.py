@@ -134,3 +134,30 @@ def test_select_states_rejects_negative_parameters():
 def truncation_select(eig, twos, kmin, kmax, terr):
     from oracle import truncation
     return truncation.select(eig, twos, kmin, kmax, terr)
+
+
+def test_fortran_descriptor_types_match_the_c_structs(tmp_path):
+    """The bind(c) mirrors in tsc_hotpath_mod.f90 have the C structs' sizes
+    (the planner and block-model descriptors a Fortran host packs)."""
+    import shutil
+    from tsc_tri_t1t2j1j2_amd import native
+    fc = shutil.which("flang") or "/opt/rocm/llvm/bin/flang"
+    if not os.path.exists(fc) and not shutil.which(fc):
+        pytest.skip("no Fortran compiler")
+    mod = os.path.join(ROOT, "tsc_tri_t1t2j1j2_amd", "fortran", "tsc_hotpath_mod.f90")
+    prog = tmp_path / "sizes.f90"
+    names = ["tsc_contrib", "tsc_outblock", "tsc_sector", "tsc_op_block", "tsc_op_desc", "tsc_pair_desc",
+             "tsc_term_desc", "tsc_sb_desc", "tsc_site_ops", "tsc_model_desc", "tsc_bond"]
+    body = "\n".join(f"  type({n}) :: v{i}" for i, n in enumerate(names))
+    prints = "\n".join(f"  print '(i0)', c_sizeof(v{i})" for i in range(len(names)))
+    prog.write_text(f"program sizes\n  use iso_c_binding\n  use tsc_hotpath\n  implicit none\n{body}\n{prints}\nend program\n")
+    exe = tmp_path / "sizes"
+    subprocess.run([fc, "-J", str(tmp_path), "-c", mod, "-o", str(tmp_path / "m.o")], check=True, cwd=tmp_path)
+    pkg = os.path.dirname(capi.LIB_PATH)
+    subprocess.run([fc, "-J", str(tmp_path), str(prog), str(tmp_path / "m.o"), "-o", str(exe), "-L", pkg,
+                    "-ltsc_hotpath", "-Wl,-rpath," + pkg], check=True, cwd=tmp_path)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [capi.CONTRIB_DTYPE.itemsize, capi.OUTBLOCK_DTYPE.itemsize] + [
+        ctypes.sizeof(t) for t in (native.Sector, native.OpBlock, native.OpDesc, native.PairDesc, native.TermDesc,
+                                   native.SbDesc, native.SiteOps, native.ModelDesc, native.Bond)]
+    assert got == want, (got, want)

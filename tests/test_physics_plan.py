@@ -116,3 +116,71 @@ def test_native_planner_matches_oracle(Lx, Ny, Ne, S2, l, order, monkeypatch):
         o, ldc, m, n = int(ob["c_off"]), int(ob["ldc"]), int(ob["m"]), int(ob["n"])
         idx = (o + np.arange(m)[:, None] + ldc * np.arange(n)[None, :]).ravel(order="F")
         assert np.allclose(acc.ravel(order="F"), got[idx], rtol=0, atol=1e-12 * max(1.0, np.abs(got).max()))
+
+
+@pytest.mark.parametrize("Lx,Ny,Ne,S2,l", CASES)
+def test_library_model_tasks_match_oracle(Lx, Ny, Ne, S2, l):
+    """tsc_model_tasks (bond classes, merged partner operators and the term
+    list in the library, the host logic of model_wave and hmerge) gives the
+    oracle's H psi from the two block models and the lattice bonds."""
+    osb, psb = _setup(Lx, Ny, Ne, S2, l)
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    nt = native.ModelTasks(psb.sys, psb.env, l + 1, l + 2, b, Ne, S2)
+    assert nt.psi_len == osb.n
+    c1, o1, c2, o2 = nt.arrays()
+    rng = np.random.default_rng(Lx * 1000 + Ny * 10 + l + 7)
+    v = rng.normal(size=osb.n) + 1j * rng.normal(size=osb.n)
+    got = plan_exec.execute(nt.arena, c1, o1, c2, o2, nt.mid_len, v, osb.n)
+    want = osb.matvec(v)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+    nt.close()
+
+
+def test_library_renorm_tasks_match_oracle():
+    """tsc_renorm_tasks (Get_Hamiltonian_Sys + update_trun in the library)
+    gives the oracle's enlarged-and-truncated block: Hamiltonian and the
+    c+, S, n of every kept site, in the new sector order."""
+    Lx, Ny = 3, 4
+    N = Lx * Ny
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    sys_blk, _, nb = dmrg_exact.build_chain_blocks(N, b, 5)
+    new_site = 6
+    to_new = [(i, t, J) for (i, t, J) in nb[new_site] if i in sys_blk.sites]
+    keep = [i for i in sys_blk.sites + [new_site] if any(j > new_site for (j, _, _) in nb.get(i, []))]
+    eb = su2_dense.enlarge(sys_blk, new_site, to_new, keep)
+    rng = np.random.default_rng(5)
+    U, order = {}, []
+    for x, (Nx, Jx, dx) in enumerate(eb.sectors):
+        if x % 3 == 1:
+            continue
+        k = max(1, dx // 2)
+        U[x] = np.linalg.qr(rng.normal(size=(dx, k)) + 1j * rng.normal(size=(dx, k)))[0]
+        order.append(x)
+    order = order[::-1]
+    want = su2_dense.truncate(eb, [U.get(x) for x in range(len(eb.sectors))])
+    bonds = [(i, new_site, t, J) for (i, t, J) in to_new]
+    rt = native.RenormTasks(block_to_product(sys_blk), new_site, bonds, keep, order,
+                            [U[x].shape[1] for x in order])
+    assert rt.enlarged_sectors == [tuple(s) for s in eb.sectors]
+    uin = np.concatenate([U[x].ravel(order="F") for x in order])
+    assert rt.in_len == uin.size
+    c1, o1, c2, o2 = rt.arrays()
+    res = plan_exec.execute(rt.arena, c1, o1, c2, o2, rt.mid_len, uin, rt.out_len)
+    okeep = [x for x in range(len(eb.sectors)) if x in U]
+    oidx = {x: n for n, x in enumerate(okeep)}
+    got = {}
+    for (key, b2, k2), off in zip(rt.layout, rt.offsets):
+        kb, kk = rt.new_sectors[b2][2], rt.new_sectors[k2][2]
+        got[(key, oidx[order[b2]], oidx[order[k2]])] = res[off:off + kb * kk].reshape((kb, kk), order="F")
+    ops = [("H", want.H)] + [((i, name), want.ops[i][name]) for i in keep for name in ("c", "S", "n")]
+    for key, (name, wop) in zip([-1] + list(range(3 * len(keep))), ops):
+        for (b2, k2), m in wop.blocks.items():
+            g = got.get((key, b2, k2))
+            if g is None:
+                assert np.abs(m).max() < 1e-13, (name, b2, k2)
+            else:
+                assert np.allclose(g, m, atol=1e-12), (name, b2, k2)
+        for (kk, b2, k2), m in got.items():
+            if kk == key and (b2, k2) not in wop.blocks:
+                assert np.abs(m).max() < 1e-13
+    rt.close()

@@ -44,6 +44,7 @@ EXPORTS = [
     "tsc_plan_autotune",
     "tsc_lanczos", "tsc_lanczos_replays", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_select_states",
     "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_flops", "tsc_tasks_copy",
+    "tsc_model_tasks", "tsc_renorm_tasks", "tsc_tasks_layout", "tsc_tasks_lengths", "tsc_tasks_copy_arena",
     "tsc_tasks_plan", "tsc_tasks_plan_dev", "tsc_tasks_destroy",
 ]
 

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <array>
 #include <tuple>
@@ -293,6 +294,17 @@ struct tsc_tasks {
   // algorithmic flops with every (L transition, R transition, source
   // window) product charged at its cheaper order and nothing shared
   double flops_pairmin = 0.0;
+  // input / output lengths of the plan (psi_len for superblocks)
+  int64_t in_len = 0, out_len = 0;
+  // operator arena built by the host logic (a copy of the caller's arena
+  // followed by merged operators); used when a plan is made without an arena
+  std::vector<double2> arena;
+  // renormalisation output layout: (operator, new bra sector, new ket
+  // sector) per window with its offset; operator -1 = Hamiltonian, else the
+  // carried operator 3 * (kept-site position) + (0 c+, 1 S, 2 n)
+  std::vector<int32_t> layout;
+  std::vector<int64_t> layout_off;
+  std::vector<int32_t> new_sec;  // (N, 2S, kept) of the new block's sectors
 };
 
 namespace {
@@ -365,6 +377,7 @@ extern "C" int tsc_superblock_tasks(const tsc_sb_desc *D, tsc_tasks **out) {
         off += (int64_t)L.sec[x].dim * R.sec[y].dim;
       }
   T->psi_len = off;
+  T->in_len = T->out_len = off;
   for (auto &e : sb) T->super_xy.insert(T->super_xy.end(), e.begin(), e.end());
   for (auto &e : L.sec) T->lsec.insert(T->lsec.end(), {e.n, e.j, e.dim});
   for (auto &e : R.sec) T->rsec.insert(T->rsec.end(), {e.n, e.j, e.dim});
@@ -624,9 +637,14 @@ extern "C" int tsc_tasks_copy(const tsc_tasks *T, tsc_contrib *c1, tsc_outblock 
 extern "C" int tsc_tasks_plan(tsc_ctx *ctx, const tsc_tasks *T, const double *arena, int64_t arena_len,
                               tsc_plan **plan) {
   TSC_REQUIRE(ctx && T && plan, "tsc_tasks_plan arguments");
+  if (!arena && !T->arena.empty()) {
+    arena = reinterpret_cast<const double *>(T->arena.data());
+    arena_len = (int64_t)T->arena.size();
+  }
+  TSC_REQUIRE(arena_len >= T->arena_len, "tsc_tasks_plan: arena shorter than the task descriptors need");
   return tsc::plan_create_impl(ctx, arena, nullptr, arena_len, T->mid_len, T->c1.data(), (int32_t)T->c1.size(),
                                T->o1.data(), (int32_t)T->o1.size(), T->c2.data(), (int32_t)T->c2.size(),
-                               T->o2.data(), (int32_t)T->o2.size(), T->psi_len, T->psi_len, plan);
+                               T->o2.data(), (int32_t)T->o2.size(), T->in_len, T->out_len, plan);
 }
 
 extern "C" int tsc_tasks_plan_dev(tsc_ctx *ctx, const tsc_tasks *T, const tsc_vec *arena, tsc_plan **plan) {
@@ -634,10 +652,468 @@ extern "C" int tsc_tasks_plan_dev(tsc_ctx *ctx, const tsc_tasks *T, const tsc_ve
   TSC_REQUIRE(arena->n >= T->arena_len, "tsc_tasks_plan_dev: arena shorter than the task descriptors need");
   return tsc::plan_create_impl(ctx, nullptr, arena->d, arena->n, T->mid_len, T->c1.data(), (int32_t)T->c1.size(),
                                T->o1.data(), (int32_t)T->o1.size(), T->c2.data(), (int32_t)T->c2.size(),
-                               T->o2.data(), (int32_t)T->o2.size(), T->psi_len, T->psi_len, plan);
+                               T->o2.data(), (int32_t)T->o2.size(), T->in_len, T->out_len, plan);
 }
 
 extern "C" int tsc_tasks_destroy(tsc_tasks *T) {
   delete T;
+  return TSC_OK;
+}
+
+// ---------------------------------------------------------------------
+// Host logic of model_wave / hmerge and of Get_Hamiltonian_Sys/Env on block
+// models: bonds of the lattice classified by the parts of the superblock
+// they join, partner operators merged (sum_j w_j O_j, hmerge,
+// get_sys.f90:7398-7626, block_add_block_two, block.f90:177-215), terms and
+// pairs formed, and the contraction tasks built by the planner above.
+namespace {
+
+constexpr double kSq2 = 1.4142135623730951;
+constexpr double kSq3 = 1.7320508075688772;
+
+// (coupling, P operator (0 c+, 1 S, 2 n), P tilde, Q operator, Q tilde) of
+// one bond: t (c+ c~ + c~ c+) / sqrt(2)-normalised hopping and
+// J (S.S - n n / 4) in reduced form; tag = 0 c+c~, 1 c~c+, 2 SS, 3 nn
+struct BondProd {
+  double g;
+  int p;
+  bool ptil;
+  int q;
+  bool qtil;
+  int tag;
+};
+
+std::vector<BondProd> bond_products(double t, double J) {
+  std::vector<BondProd> out;
+  if (t != 0.0) {
+    out.push_back(BondProd{t * kSq2, 0, false, 0, true, 0});
+    out.push_back(BondProd{t * kSq2, 0, true, 0, false, 1});
+  }
+  if (J != 0.0) {
+    out.push_back(BondProd{-kSq3 * J, 1, false, 1, false, 2});
+    out.push_back(BondProd{-0.25 * J, 2, false, 2, false, 3});
+  }
+  return out;
+}
+
+// operators made by the host logic (merged sums), kept alive with their blocks
+struct OpStore {
+  std::deque<std::vector<tsc_op_block>> blks;
+  std::deque<tsc_op_desc> descs;
+  const tsc_op_desc *add(int side, int up_dif, int k2, std::vector<tsc_op_block> b) {
+    blks.push_back(std::move(b));
+    tsc_op_desc d;
+    d.side = side;
+    d.up_dif = up_dif;
+    d.k2 = k2;
+    d.nblk = (int32_t)blks.back().size();
+    d.blk = blks.back().data();
+    descs.push_back(d);
+    return &descs.back();
+  }
+};
+
+// sum_j w_j O_j over the blocks of the operators (union of their sub-blocks,
+// in order of first appearance), appended to the arena
+const tsc_op_desc *merge_ops(OpStore &st, std::vector<double2> &arena, const std::vector<tsc_sector> &sec,
+                             const std::vector<std::pair<const tsc_op_desc *, double>> &parts) {
+  std::vector<tsc_op_block> out;
+  std::map<std::pair<int, int>, int64_t> where;
+  for (const auto &pw : parts)
+    for (int i = 0; i < pw.first->nblk; ++i) {
+      const tsc_op_block &b = pw.first->blk[i];
+      if (where.count({b.bra, b.ket})) continue;
+      const int64_t off = (int64_t)arena.size();
+      arena.resize(arena.size() + (size_t)sec[b.bra].dim * sec[b.ket].dim, make_double2(0.0, 0.0));
+      where[{b.bra, b.ket}] = off;
+      out.push_back(tsc_op_block{b.bra, b.ket, off});
+    }
+  for (const auto &pw : parts)
+    for (int i = 0; i < pw.first->nblk; ++i) {
+      const tsc_op_block &b = pw.first->blk[i];
+      const int64_t dst = where[{b.bra, b.ket}], n = (int64_t)sec[b.bra].dim * sec[b.ket].dim;
+      for (int64_t e = 0; e < n; ++e) {
+        arena[dst + e].x += pw.second * arena[b.off + e].x;
+        arena[dst + e].y += pw.second * arena[b.off + e].y;
+      }
+    }
+  const tsc_op_desc *o = parts.front().first;
+  return st.add(o->side, o->up_dif, o->k2, std::move(out));
+}
+
+// operator `which` (0 c+, 1 S, 2 n) of a kept site of a model, or null
+const tsc_op_desc *site_op(const tsc_model_desc *m, int site, int which) {
+  for (int i = 0; i < m->nkeep; ++i)
+    if (m->keep[i].site == site) return which == 0 ? m->keep[i].c : (which == 1 ? m->keep[i].S : m->keep[i].n);
+  return nullptr;
+}
+
+int check_model(const tsc_model_desc *m, int64_t arena_len, const char *what) {
+  TSC_REQUIRE(m && m->nsec >= 0 && (m->nsec == 0 || m->sec) && m->ham && m->nsites >= 0 && m->nkeep >= 0,
+              std::string(what) + " model descriptor");
+  auto check_op = [&](const tsc_op_desc *op) -> int {
+    TSC_REQUIRE(op && op->nblk >= 0 && (op->nblk == 0 || op->blk), std::string(what) + " operator descriptor");
+    for (int b = 0; b < op->nblk; ++b) {
+      const tsc_op_block &bl = op->blk[b];
+      TSC_REQUIRE(bl.bra >= 0 && bl.bra < m->nsec && bl.ket >= 0 && bl.ket < m->nsec,
+                  std::string(what) + " operator block sector index");
+      TSC_REQUIRE(bl.off >= 0 && bl.off + (int64_t)m->sec[bl.bra].dim * m->sec[bl.ket].dim <= arena_len,
+                  std::string(what) + " operator block outside the arena");
+    }
+    return TSC_OK;
+  };
+  int rc = check_op(m->ham);
+  if (rc) return rc;
+  for (int i = 0; i < m->nkeep; ++i)
+    for (const tsc_op_desc *op : {m->keep[i].c, m->keep[i].S, m->keep[i].n})
+      if ((rc = check_op(op))) return rc;
+  return TSC_OK;
+}
+
+}  // namespace
+
+extern "C" int tsc_model_tasks(const tsc_model_desc *sys, const tsc_model_desc *env, int32_t s1, int32_t s2,
+                               int32_t nbonds, const tsc_bond *bonds, int32_t ntot, int32_t s2tot,
+                               const double *arena, int64_t arena_len, tsc_tasks **out) {
+  TSC_REQUIRE(out && nbonds >= 0 && (nbonds == 0 || bonds) && arena_len >= 0 && (arena || arena_len == 0),
+              "tsc_model_tasks arguments");
+  int rc = check_model(sys, arena_len, "sys");
+  if (!rc) rc = check_model(env, arena_len, "env");
+  if (rc) return rc;
+  std::vector<double2> ar(reinterpret_cast<const double2 *>(arena),
+                          reinterpret_cast<const double2 *>(arena) + arena_len);
+  std::vector<tsc_sector> A(sys->sec, sys->sec + sys->nsec), B(env->sec, env->sec + env->nsec);
+  // parts: 0 A, 1 s1, 2 B, 3 s2 (the order classes are written in)
+  std::map<int, int> part;
+  for (int i = 0; i < sys->nsites; ++i) part[sys->sites[i]] = 0;
+  for (int i = 0; i < env->nsites; ++i) part[env->sites[i]] = 2;
+  part[s1] = 1;
+  part[s2] = 3;
+  std::map<std::pair<int, int>, std::vector<tsc_bond>> cls;
+  for (int b = 0; b < nbonds; ++b) {
+    tsc_bond bd = bonds[b];
+    auto pi = part.find(bd.i), pj = part.find(bd.j);
+    if (pi == part.end() || pj == part.end() || pi->second == pj->second) continue;
+    int a = pi->second, c = pj->second;
+    if (a > c) {
+      std::swap(bd.i, bd.j);
+      std::swap(a, c);
+    }
+    cls[{a, c}].push_back(bd);
+  }
+  OpStore st;
+  std::vector<tsc_op_desc> ops;   // every operator the descriptors address
+  auto op_index = [&](const tsc_op_desc *d, int side) {
+    tsc_op_desc c = *d;
+    c.side = side;
+    ops.push_back(c);
+    return (int32_t)ops.size() - 1;
+  };
+  auto need = [&](const tsc_model_desc *m, int site, int which) -> const tsc_op_desc * {
+    const tsc_op_desc *o = site_op(m, site, which);
+    if (!o) set_error("invalid: block site " + std::to_string(site) + " has a bond outside its block but no operators");
+    return o;
+  };
+  std::vector<tsc_pair_desc> lpairs, rpairs;
+  std::vector<tsc_term_desc> terms;
+  // (A, s1) and (B, s2): block-site pairs of one side, merged over block sites
+  for (int side = 0; side < 2; ++side) {
+    const tsc_model_desc *m = side == 0 ? sys : env;
+    auto it = cls.find(side == 0 ? std::make_pair(0, 1) : std::make_pair(2, 3));
+    if (it == cls.end()) continue;
+    for (int tag = 0; tag < 4; ++tag) {
+      std::vector<std::pair<const tsc_op_desc *, double>> parts;
+      BondProd spec{};
+      for (const tsc_bond &bd : it->second)
+        for (const BondProd &bp : bond_products(bd.t, bd.J))
+          if (bp.tag == tag) {
+            const tsc_op_desc *o = need(m, bd.i, bp.p);
+            if (!o) return TSC_EINVAL;
+            parts.push_back({o, bp.g});
+            spec = bp;
+          }
+      if (parts.empty()) continue;
+      const tsc_op_desc *M = merge_ops(st, ar, side == 0 ? A : B, parts);
+      const tsc_pair_desc pd{1.0, op_index(M, side), spec.ptil ? 1 : 0, spec.q, spec.qtil ? 1 : 0};
+      (side == 0 ? lpairs : rpairs).push_back(pd);
+    }
+  }
+  // L-R classes in the order (A, B), (A, s2), (s1, B), (s1, s2)
+  for (auto key : {std::make_pair(0, 2), std::make_pair(0, 3), std::make_pair(1, 2), std::make_pair(1, 3)}) {
+    auto it = cls.find(key);
+    if (it == cls.end()) continue;
+    const std::vector<tsc_bond> &bl = it->second;
+    if (key == std::make_pair(0, 2)) {
+      // one term per A site, its B partners merged
+      std::map<int, std::vector<tsc_bond>> by_i;
+      for (const tsc_bond &bd : bl) by_i[bd.i].push_back(bd);
+      for (auto &kv : by_i)
+        for (int tag = 0; tag < 4; ++tag) {
+          std::vector<std::pair<const tsc_op_desc *, double>> parts;
+          BondProd spec{};
+          for (const tsc_bond &bd : kv.second)
+            for (const BondProd &bp : bond_products(bd.t, bd.J))
+              if (bp.tag == tag) {
+                const tsc_op_desc *o = need(env, bd.j, bp.q);
+                if (!o) return TSC_EINVAL;
+                parts.push_back({o, bp.g});
+                spec = bp;
+              }
+          if (parts.empty()) continue;
+          const tsc_op_desc *P = need(sys, kv.first, spec.p);
+          if (!P) return TSC_EINVAL;
+          const tsc_op_desc *MB = merge_ops(st, ar, B, parts);
+          terms.push_back(tsc_term_desc{1.0, 1, op_index(P, 0), spec.ptil ? 1 : 0, 1, op_index(MB, 1),
+                                        spec.qtil ? 1 : 0});
+        }
+      continue;
+    }
+    for (int tag = 0; tag < 4; ++tag) {
+      std::vector<std::pair<const tsc_op_desc *, double>> parts;
+      double gsum = 0.0;
+      BondProd spec{};
+      bool any = false;
+      for (const tsc_bond &bd : bl)
+        for (const BondProd &bp : bond_products(bd.t, bd.J))
+          if (bp.tag == tag) {
+            any = true;
+            spec = bp;
+            gsum += bp.g;
+            if (key.first == 0) {   // (A, s2): merge over A sites
+              const tsc_op_desc *o = need(sys, bd.i, bp.p);
+              if (!o) return TSC_EINVAL;
+              parts.push_back({o, bp.g});
+            } else if (key.second == 2) {   // (s1, B): merge over B sites
+              const tsc_op_desc *o = need(env, bd.j, bp.q);
+              if (!o) return TSC_EINVAL;
+              parts.push_back({o, bp.g});
+            }
+          }
+      if (!any) continue;
+      if (key.first == 0) {
+        const tsc_op_desc *M = merge_ops(st, ar, A, parts);
+        terms.push_back(tsc_term_desc{1.0, 1, op_index(M, 0), spec.ptil ? 1 : 0, 2, spec.q, spec.qtil ? 1 : 0});
+      } else if (key.second == 2) {
+        const tsc_op_desc *M = merge_ops(st, ar, B, parts);
+        terms.push_back(tsc_term_desc{1.0, 2, spec.p, spec.ptil ? 1 : 0, 1, op_index(M, 1), spec.qtil ? 1 : 0});
+      } else {
+        terms.push_back(tsc_term_desc{gsum, 2, spec.p, spec.ptil ? 1 : 0, 2, spec.q, spec.qtil ? 1 : 0});
+      }
+    }
+  }
+  const int32_t hs = op_index(sys->ham, 0), he = op_index(env->ham, 1);
+  tsc_sb_desc D;
+  std::memset(&D, 0, sizeof(D));
+  D.nsys = sys->nsec;
+  D.nenv = env->nsec;
+  D.sys = sys->sec;
+  D.env = env->sec;
+  D.arena_len = (int64_t)ar.size();
+  D.nops = (int32_t)ops.size();
+  D.ops = ops.data();
+  D.sys_ham = &ops[hs];
+  D.env_ham = &ops[he];
+  D.nlpairs = (int32_t)lpairs.size();
+  D.lpairs = lpairs.data();
+  D.nrpairs = (int32_t)rpairs.size();
+  D.rpairs = rpairs.data();
+  D.nterms = (int32_t)terms.size();
+  D.terms = terms.data();
+  D.ntot = ntot;
+  D.s2tot = s2tot;
+  tsc_tasks *T = nullptr;
+  rc = tsc_superblock_tasks(&D, &T);
+  if (rc) return rc;
+  T->arena = std::move(ar);
+  *out = T;
+  return TSC_OK;
+}
+
+extern "C" int tsc_renorm_tasks(const tsc_model_desc *blk, int32_t new_site, int32_t nbonds, const tsc_bond *bonds,
+                                int32_t nkeep, const int32_t *keep_sites, int32_t norder, const int32_t *order,
+                                const int32_t *kept, const double *arena, int64_t arena_len, tsc_tasks **out) {
+  TSC_REQUIRE(out && nbonds >= 0 && (nbonds == 0 || bonds) && nkeep >= 0 && (nkeep == 0 || keep_sites) &&
+                  norder >= 0 && (norder == 0 || (order && kept)) && arena_len >= 0 && (arena || arena_len == 0),
+              "tsc_renorm_tasks arguments");
+  int rc = check_model(blk, arena_len, "block");
+  if (rc) return rc;
+  std::vector<double2> ar(reinterpret_cast<const double2 *>(arena),
+                          reinterpret_cast<const double2 *>(arena) + arena_len);
+  std::vector<tsc_sector> sec(blk->sec, blk->sec + blk->nsec);
+  const Basis basis = get_basis(sec);
+  const int nx = (int)basis.sec.size();
+  std::vector<int> kx(nx, -1), newidx(nx, -1);
+  std::vector<int64_t> uo(nx, 0);
+  int64_t ulen = 0;
+  for (int q = 0; q < norder; ++q) {
+    const int x = order[q];
+    TSC_REQUIRE(x >= 0 && x < nx && kx[x] < 0, "order: enlarged sector index out of range or repeated");
+    TSC_REQUIRE(kept[q] >= 1 && kept[q] <= basis.sec[x].dim, "kept states of a sector outside [1, dim]");
+    kx[x] = kept[q];
+    newidx[x] = q;
+    uo[x] = ulen;
+    ulen += (int64_t)basis.sec[x].dim * kept[q];
+  }
+  // merged block-site pairs of the enlarged block's Hamiltonian
+  OpStore st;
+  std::vector<tsc_op_desc> ops;
+  std::vector<tsc_pair_desc> pairs;
+  for (int tag = 0; tag < 4; ++tag) {
+    std::vector<std::pair<const tsc_op_desc *, double>> parts;
+    BondProd spec{};
+    for (int b = 0; b < nbonds; ++b) {
+      int i = bonds[b].i, j = bonds[b].j;
+      if (j != new_site) std::swap(i, j);
+      if (j != new_site || i == new_site) continue;
+      bool inside = false;
+      for (int k = 0; k < blk->nsites; ++k) inside |= blk->sites[k] == i;
+      if (!inside) continue;
+      for (const BondProd &bp : bond_products(bonds[b].t, bonds[b].J))
+        if (bp.tag == tag) {
+          const tsc_op_desc *o = site_op(blk, i, bp.p);
+          TSC_REQUIRE(o, "block site " + std::to_string(i) + " bonded to the new site has no operators");
+          parts.push_back({o, bp.g});
+          spec = bp;
+        }
+    }
+    if (parts.empty()) continue;
+    const tsc_op_desc *M = merge_ops(st, ar, sec, parts);
+    ops.push_back(*M);
+    pairs.push_back(tsc_pair_desc{1.0, (int32_t)ops.size() - 1, spec.ptil ? 1 : 0, spec.q, spec.qtil ? 1 : 0});
+  }
+  tsc_sb_desc D;
+  std::memset(&D, 0, sizeof(D));
+  D.arena_len = (int64_t)ar.size();
+  Ctx cx;
+  cx.D = &D;
+  // transitions: H, then the carried operators of every kept site
+  std::vector<std::vector<Tr>> trans;
+  std::vector<int32_t> opkey;
+  trans.emplace_back();
+  opkey.push_back(-1);
+  scalar_transitions(cx, basis, sec, blk->ham, pairs.data(), (int)pairs.size(), ops.data(), true, trans.back());
+  for (int k = 0; k < nkeep; ++k)
+    for (int which = 0; which < 3; ++which) {
+      trans.emplace_back();
+      opkey.push_back(3 * k + which);
+      if (keep_sites[k] == new_site) {
+        site_transitions(cx, basis, sec, which, false, trans.back());
+      } else {
+        const tsc_op_desc *o = site_op(blk, keep_sites[k], which);
+        TSC_REQUIRE(o, "kept site " + std::to_string(keep_sites[k]) + " has no operators in the block");
+        blk_transitions(cx, basis, sec, *o, false, trans.back());
+      }
+    }
+  tsc_tasks *T = new tsc_tasks();
+  T->arena_len = (int64_t)ar.size();
+  T->in_len = ulen;
+  // output windows (operator, x2, x) in first-appearance order
+  std::map<std::tuple<int, int, int>, int> wid;
+  std::vector<std::vector<tsc_contrib>> per_w;
+  std::map<std::tuple<int, int, int>, int64_t> mid;   // (matrix, x, p) -> MID offset of T U_x[rows]
+  int64_t wout = 0;
+  auto window = [&](int key, int x2, int x) {
+    auto k = std::make_tuple(key, x2, x);
+    auto it = wid.find(k);
+    if (it != wid.end()) return it->second;
+    tsc_outblock ob;
+    std::memset(&ob, 0, sizeof(ob));
+    ob.c_off = wout;
+    ob.ldc = std::max(1, kx[x2]);
+    ob.m = kx[x2];
+    ob.n = kx[x];
+    ob.c_buf = TSC_BUF_OUT;
+    T->o2.push_back(ob);
+    per_w.emplace_back();
+    T->layout.insert(T->layout.end(), {key, newidx[x2], newidx[x]});
+    T->layout_off.push_back(wout);
+    wout += (int64_t)kx[x2] * kx[x];
+    const int w = (int)T->o2.size() - 1;
+    wid[k] = w;
+    return w;
+  };
+  for (size_t o = 0; o < trans.size(); ++o)
+    for (const Tr &t : trans[o]) {
+      if (kx[t.x] < 0 || kx[t.x2] < 0 || t.f == 0.0) continue;
+      const int w = window(opkey[o], t.x2, t.x);
+      const int dx = basis.sec[t.x].dim, dx2 = basis.sec[t.x2].dim;
+      if (t.mat >= 0) {
+        const MatRef &m = cx.mats[t.mat];
+        auto key = std::make_tuple(t.mat, t.x, t.p);
+        int64_t moff;
+        auto it = mid.find(key);
+        if (it != mid.end()) {
+          moff = it->second;
+        } else {
+          moff = T->mid_len;
+          T->mid_len += (int64_t)t.d2 * kx[t.x];
+          mid[key] = moff;
+          tsc_outblock ob;
+          std::memset(&ob, 0, sizeof(ob));
+          ob.c_off = moff;
+          ob.ldc = std::max(1, t.d2);
+          ob.m = t.d2;
+          ob.n = kx[t.x];
+          ob.c_buf = TSC_BUF_MID;
+          ob.cbeg = (int32_t)T->c1.size();
+          ob.cend = ob.cbeg + 1;
+          T->o1.push_back(ob);
+          T->c1.push_back(Builder::make(m.off, m.ld, TSC_BUF_OPS, m.op, uo[t.x] + t.p, std::max(1, dx), TSC_BUF_IN,
+                                        TSC_OP_N, t.d, 1.0));
+        }
+        per_w[w].push_back(Builder::make(uo[t.x2] + t.p2, std::max(1, dx2), TSC_BUF_IN, TSC_OP_C, moff,
+                                         std::max(1, t.d2), TSC_BUF_MID, TSC_OP_N, t.d2, t.f));
+      } else {
+        per_w[w].push_back(Builder::make(uo[t.x2] + t.p2, std::max(1, dx2), TSC_BUF_IN, TSC_OP_C, uo[t.x] + t.p,
+                                         std::max(1, dx), TSC_BUF_IN, TSC_OP_N, t.d, t.f));
+      }
+    }
+  for (size_t w = 0; w < T->o2.size(); ++w) {
+    T->o2[w].cbeg = (int32_t)T->c2.size();
+    T->c2.insert(T->c2.end(), per_w[w].begin(), per_w[w].end());
+    T->o2[w].cend = (int32_t)T->c2.size();
+  }
+  // diagonal Hamiltonian blocks that no term reaches: zero windows (not
+  // written by the plan; the output vector starts zeroed)
+  for (int q = 0; q < norder; ++q) {
+    const int x = order[q];
+    if (wid.count(std::make_tuple(-1, x, x))) continue;
+    T->layout.insert(T->layout.end(), {-1, q, q});
+    T->layout_off.push_back(wout);
+    wout += (int64_t)kx[x] * kx[x];
+  }
+  T->out_len = std::max<int64_t>(wout, 1);
+  for (int q = 0; q < norder; ++q)
+    T->new_sec.insert(T->new_sec.end(), {basis.sec[order[q]].n, basis.sec[order[q]].j, kx[order[q]]});
+  for (const EnlSector &e : basis.sec) T->lsec.insert(T->lsec.end(), {e.n, e.j, e.dim});
+  T->arena = std::move(ar);
+  *out = T;
+  return TSC_OK;
+}
+
+extern "C" int tsc_tasks_layout(const tsc_tasks *T, int64_t *counts3, int32_t *layout, int64_t *offsets,
+                                int32_t *new_sec, int32_t *enl_sec) {
+  TSC_REQUIRE(T && counts3, "tsc_tasks_layout arguments");
+  counts3[0] = (int64_t)T->layout_off.size();
+  counts3[1] = (int64_t)T->new_sec.size() / 3;
+  counts3[2] = (int64_t)T->lsec.size() / 3;
+  if (layout) std::copy(T->layout.begin(), T->layout.end(), layout);
+  if (offsets) std::copy(T->layout_off.begin(), T->layout_off.end(), offsets);
+  if (new_sec) std::copy(T->new_sec.begin(), T->new_sec.end(), new_sec);
+  if (enl_sec) std::copy(T->lsec.begin(), T->lsec.end(), enl_sec);
+  return TSC_OK;
+}
+
+extern "C" int tsc_tasks_lengths(const tsc_tasks *T, int64_t *lens3) {
+  TSC_REQUIRE(T && lens3, "tsc_tasks_lengths arguments");
+  lens3[0] = T->in_len;
+  lens3[1] = T->out_len;
+  lens3[2] = (int64_t)T->arena.size();
+  return TSC_OK;
+}
+
+extern "C" int tsc_tasks_copy_arena(const tsc_tasks *T, double *arena_out) {
+  TSC_REQUIRE(T && (arena_out || T->arena.empty()), "tsc_tasks_copy_arena arguments");
+  if (!T->arena.empty()) std::memcpy(arena_out, T->arena.data(), sizeof(double2) * T->arena.size());
   return TSC_OK;
 }

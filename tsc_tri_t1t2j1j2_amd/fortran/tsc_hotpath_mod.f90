@@ -30,6 +30,66 @@ module tsc_hotpath
     integer(c_int32_t) :: c_buf, cbeg, cend, mode
   end type tsc_outblock
 
+  ! descriptors of the SU(2) planner and of block models (C structs of the
+  ! same names in tsc_hotpath.h, byte for byte)
+  type, bind(c) :: tsc_sector
+    integer(c_int32_t) :: n, twos, dim
+  end type tsc_sector
+
+  type, bind(c) :: tsc_op_block
+    integer(c_int32_t) :: bra, ket
+    integer(c_int64_t) :: off
+  end type tsc_op_block
+
+  type, bind(c) :: tsc_op_desc
+    integer(c_int32_t) :: side, up_dif, k2, nblk
+    type(c_ptr) :: blk
+  end type tsc_op_desc
+
+  type, bind(c) :: tsc_pair_desc
+    real(c_double) :: coef
+    integer(c_int32_t) :: block_op, block_tilde, site_op, site_tilde
+  end type tsc_pair_desc
+
+  type, bind(c) :: tsc_term_desc
+    real(c_double) :: coef
+    integer(c_int32_t) :: l_kind, l_op, l_tilde, r_kind, r_op, r_tilde
+  end type tsc_term_desc
+
+  type, bind(c) :: tsc_sb_desc
+    integer(c_int32_t) :: nsys, nenv
+    type(c_ptr) :: sys, env
+    integer(c_int64_t) :: arena_len
+    integer(c_int32_t) :: nops
+    type(c_ptr) :: ops, sys_ham, env_ham
+    integer(c_int32_t) :: nlpairs
+    type(c_ptr) :: lpairs
+    integer(c_int32_t) :: nrpairs
+    type(c_ptr) :: rpairs
+    integer(c_int32_t) :: nterms
+    type(c_ptr) :: terms
+    integer(c_int32_t) :: ntot, s2tot
+  end type tsc_sb_desc
+
+  type, bind(c) :: tsc_site_ops
+    integer(c_int32_t) :: site
+    type(c_ptr) :: op_c, op_s, op_n
+  end type tsc_site_ops
+
+  type, bind(c) :: tsc_model_desc
+    integer(c_int32_t) :: nsec
+    type(c_ptr) :: sec
+    integer(c_int32_t) :: nsites
+    type(c_ptr) :: sites, ham
+    integer(c_int32_t) :: nkeep
+    type(c_ptr) :: keep
+  end type tsc_model_desc
+
+  type, bind(c) :: tsc_bond
+    integer(c_int32_t) :: i, j
+    real(c_double) :: t, jex
+  end type tsc_bond
+
   interface
     function tsc_last_error() bind(c, name="tsc_last_error") result(p)
       import :: c_ptr
@@ -247,6 +307,35 @@ module tsc_hotpath
     integer(c_int) function tsc_tasks_flops(tasks, flops2) bind(c, name="tsc_tasks_flops")
       import :: c_int, c_ptr
       type(c_ptr), value :: tasks, flops2
+    end function
+    integer(c_int) function tsc_model_tasks(sys, env, s1, s2, nbonds, bonds, ntot, s2tot, arena, arena_len, &
+                                            tasks) bind(c, name="tsc_model_tasks")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: sys, env, bonds, arena
+      integer(c_int32_t), value :: s1, s2, nbonds, ntot, s2tot
+      integer(c_int64_t), value :: arena_len
+      type(c_ptr) :: tasks
+    end function
+    integer(c_int) function tsc_renorm_tasks(blk, new_site, nbonds, bonds, nkeep, keep_sites, norder, order, kept, &
+                                             arena, arena_len, tasks) bind(c, name="tsc_renorm_tasks")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: blk, bonds, keep_sites, order, kept, arena
+      integer(c_int32_t), value :: new_site, nbonds, nkeep, norder
+      integer(c_int64_t), value :: arena_len
+      type(c_ptr) :: tasks
+    end function
+    integer(c_int) function tsc_tasks_layout(tasks, counts3, layout, offsets, new_sec, enl_sec) &
+                                             bind(c, name="tsc_tasks_layout")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tasks, counts3, layout, offsets, new_sec, enl_sec
+    end function
+    integer(c_int) function tsc_tasks_lengths(tasks, lens3) bind(c, name="tsc_tasks_lengths")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tasks, lens3
+    end function
+    integer(c_int) function tsc_tasks_copy_arena(tasks, arena) bind(c, name="tsc_tasks_copy_arena")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: tasks, arena
     end function
     integer(c_int) function tsc_superblock_tasks(desc, tasks) bind(c, name="tsc_superblock_tasks")
       import :: c_int, c_ptr

@@ -43,6 +43,19 @@ class SbDesc(ctypes.Structure):
                 ("s2tot", c_int32)]
 
 
+class SiteOps(ctypes.Structure):
+    _fields_ = [("site", c_int32), ("c", P(OpDesc)), ("S", P(OpDesc)), ("n", P(OpDesc))]
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [("nsec", c_int32), ("sec", P(Sector)), ("nsites", c_int32), ("sites", P(c_int32)),
+                ("ham", P(OpDesc)), ("nkeep", c_int32), ("keep", P(SiteOps))]
+
+
+class Bond(ctypes.Structure):
+    _fields_ = [("i", c_int32), ("j", c_int32), ("t", c_double), ("J", c_double)]
+
+
 _SITE_ID = {"c": 0, "S": 1, "n": 2}
 
 
@@ -54,6 +67,18 @@ def _bind():
     L.tsc_superblock_tasks.restype = ctypes.c_int
     L.tsc_tasks_info.argtypes = [ctypes.c_void_p, P(c_int64)]
     L.tsc_tasks_info.restype = ctypes.c_int
+    L.tsc_model_tasks.argtypes = [P(ModelDesc), P(ModelDesc), c_int32, c_int32, c_int32, P(Bond), c_int32,
+                                  c_int32, ctypes.c_void_p, c_int64, P(ctypes.c_void_p)]
+    L.tsc_model_tasks.restype = ctypes.c_int
+    L.tsc_renorm_tasks.argtypes = [P(ModelDesc), c_int32, c_int32, P(Bond), c_int32, P(c_int32), c_int32,
+                                   P(c_int32), P(c_int32), ctypes.c_void_p, c_int64, P(ctypes.c_void_p)]
+    L.tsc_renorm_tasks.restype = ctypes.c_int
+    L.tsc_tasks_layout.argtypes = [ctypes.c_void_p, P(c_int64)] + [ctypes.c_void_p] * 4
+    L.tsc_tasks_layout.restype = ctypes.c_int
+    L.tsc_tasks_lengths.argtypes = [ctypes.c_void_p, P(c_int64)]
+    L.tsc_tasks_lengths.restype = ctypes.c_int
+    L.tsc_tasks_copy_arena.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.tsc_tasks_copy_arena.restype = ctypes.c_int
     L.tsc_tasks_flops.argtypes = [ctypes.c_void_p, P(c_double)]
     L.tsc_tasks_flops.restype = ctypes.c_int
     L.tsc_tasks_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 7
@@ -184,3 +209,125 @@ class NativeTasks:
             self.close()
         except Exception:
             pass
+
+
+class _ModelPack:
+    """A TotalModel with host blocks packed for the library: one arena
+    (shared by several models), sector list, operator descriptors."""
+
+    def __init__(self, model, arena, side):
+        self._keep = []
+        self.sec = (Sector * max(len(model.sectors), 1))(*[Sector(*s) for s in model.sectors])
+        self.sites = (c_int32 * max(len(model.sites), 1))(*model.sites)
+
+        def op(tb):
+            items = list(tb.blocks.items())
+            blks = (OpBlock * max(len(items), 1))()
+            for n, ((b, k), m) in enumerate(items):
+                m = np.asarray(m, np.complex128)
+                blks[n].bra, blks[n].ket, blks[n].off = b, k, arena.add(m)
+            self._keep.append(blks)
+            d = OpDesc(side, tb.up_dif, tb.k2, len(items), blks)
+            self._keep.append(d)
+            return ctypes.pointer(d)
+
+        self.ham = op(model.ham)
+        keep = sorted(model.ops)
+        self.keep = (SiteOps * max(len(keep), 1))()
+        for n, i in enumerate(keep):
+            o = model.ops[i]
+            self.keep[n] = SiteOps(i, op(o["c"]), op(o["S"]), op(o["n"]))
+        self.desc = ModelDesc(len(model.sectors), self.sec, len(model.sites), self.sites, self.ham, len(keep),
+                              self.keep)
+
+
+class _Arena:
+    def __init__(self):
+        self.parts, self.n = [], 0
+
+    def add(self, m):
+        off = self.n
+        self.parts.append(m.ravel(order="F"))
+        self.n += m.size
+        return off
+
+    def array(self):
+        return np.concatenate(self.parts) if self.parts else np.zeros(1, np.complex128)
+
+
+def _bonds(bonds):
+    arr = (Bond * max(len(bonds), 1))()
+    for n, (i, j, t, J) in enumerate(bonds):
+        arr[n] = Bond(int(i), int(j), float(t), float(J))
+    return arr, len(bonds)
+
+
+class ModelTasks(NativeTasks):
+    """Task arrays of a superblock built by the library from the two block
+    models and the lattice bonds (tsc_model_tasks: term list and merged
+    partner operators in the library)."""
+
+    def __init__(self, sys, env, s1, s2, bonds, ntot, s2tot):
+        L = _bind()
+        ar = _Arena()
+        ps, pe = _ModelPack(sys, ar, 0), _ModelPack(env, ar, 1)
+        arena = ar.array()
+        b, nb = _bonds(bonds)
+        h = ctypes.c_void_p()
+        capi.check(L.tsc_model_tasks(ctypes.byref(ps.desc), ctypes.byref(pe.desc), int(s1), int(s2), nb, b,
+                                     int(ntot), int(s2tot), arena.ctypes.data_as(ctypes.c_void_p), arena.size,
+                                     ctypes.byref(h)))
+        self.h = h
+        self.step = None
+        lens = (c_int64 * 3)()
+        capi.check(L.tsc_tasks_lengths(h, lens))
+        self.arena = np.empty(max(lens[2], 1), np.complex128)
+        capi.check(L.tsc_tasks_copy_arena(h, self.arena.ctypes.data_as(ctypes.c_void_p)))
+        self.arena_len = int(lens[2])
+        cnt = (c_int64 * 9)()
+        capi.check(L.tsc_tasks_info(h, cnt))
+        self.counts = list(cnt)
+        self.mid_len, self.psi_len = cnt[4], cnt[5]
+
+
+class RenormTasks(NativeTasks):
+    """Enlargement + truncation of a block built by the library
+    (tsc_renorm_tasks); in_len / out_len and the output layout."""
+
+    def __init__(self, blk, new_site, bonds, keep_sites, order, kept):
+        L = _bind()
+        ar = _Arena()
+        pb = _ModelPack(blk, ar, 0)
+        arena = ar.array()
+        b, nb = _bonds(bonds)
+        ks = (c_int32 * max(len(keep_sites), 1))(*keep_sites)
+        od = (c_int32 * max(len(order), 1))(*order)
+        kp = (c_int32 * max(len(kept), 1))(*kept)
+        h = ctypes.c_void_p()
+        capi.check(L.tsc_renorm_tasks(ctypes.byref(pb.desc), int(new_site), nb, b, len(keep_sites), ks, len(order),
+                                      od, kp, arena.ctypes.data_as(ctypes.c_void_p), arena.size, ctypes.byref(h)))
+        self.h = h
+        self.step = None
+        lens = (c_int64 * 3)()
+        capi.check(L.tsc_tasks_lengths(h, lens))
+        self.in_len, self.out_len = int(lens[0]), int(lens[1])
+        self.arena = np.empty(max(lens[2], 1), np.complex128)
+        capi.check(L.tsc_tasks_copy_arena(h, self.arena.ctypes.data_as(ctypes.c_void_p)))
+        self.arena_len = int(lens[2])
+        cnt = (c_int64 * 9)()
+        capi.check(L.tsc_tasks_info(h, cnt))
+        self.counts = list(cnt)
+        self.mid_len = cnt[4]
+        c3 = (c_int64 * 3)()
+        capi.check(L.tsc_tasks_layout(h, c3, None, None, None, None))
+        nw, ns, ne = c3
+        lay = np.zeros(3 * max(nw, 1), np.int32)
+        off = np.zeros(max(nw, 1), np.int64)
+        nsec = np.zeros(3 * max(ns, 1), np.int32)
+        esec = np.zeros(3 * max(ne, 1), np.int32)
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        capi.check(L.tsc_tasks_layout(h, c3, ptr(lay), ptr(off), ptr(nsec), ptr(esec)))
+        self.layout = [tuple(int(v) for v in lay[3 * w:3 * w + 3]) for w in range(nw)]
+        self.offsets = [int(v) for v in off[:nw]]
+        self.new_sectors = [tuple(int(v) for v in nsec[3 * q:3 * q + 3]) for q in range(ns)]
+        self.enlarged_sectors = [tuple(int(v) for v in esec[3 * q:3 * q + 3]) for q in range(ne)]
