@@ -2,7 +2,7 @@
 
 Default workload: the YC6 6x24 t1-t2-J1-J2 cylinder (144 sites, 132
 electrons, t2 = 0.3, J2 = 0.036) at M = 6000 multiplets. A DMRG warm-up
-grows the blocks to the centre (kept_max 2000, its last six steps 6000),
+grows the blocks to the centre (kept_max 1000, its last six steps 6000),
 then the first half-sweep's centre steps run twice from the same blocks:
 mixed precision (BF16x3 H.psi, FP64 Lanczos and refinement) and FP64. At
 the first centre step, before its solve, the H.psi of that real superblock
@@ -50,9 +50,9 @@ def parse():
                     help="yc6-m6000: real DMRG superblocks (warm-up, centre window); *-hist: the real centre "
                          "sector histogram (x2 per sector for M=12000) with seeded random block values, H.psi only")
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "f32", "f64"])
-    ap.add_argument("--window-steps", type=int, default=3, help="centre steps timed per precision")
-    ap.add_argument("--ramp", default="2000,6", help="warm-up kept_max and the number of final steps at M "
-                                                      "(at least the window steps + 3, so that every block the "
+    ap.add_argument("--window-steps", type=int, default=2, help="centre steps timed per precision")
+    ap.add_argument("--ramp", default="1000,6", help="warm-up kept_max and the number of final steps at M "
+                                                      "(at least the window steps + 4, so that every block the "
                                                       "window reads was grown from an M-block)")
     ap.add_argument("--warmup-lanzero", type=float, default=1e-6)
     ap.add_argument("--window-lanzero", type=float, default=1e-8)
@@ -60,7 +60,7 @@ def parse():
                                                            "(its first H.psi FP64)")
     ap.add_argument("--no-synthetic", action="store_true", help="skip the isolated 400x24 H.psi superblock")
     ap.add_argument("--no-window", action="store_true", help="real run: time H.psi only, no centre window")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the measurement, copy a buffer of this many MiB three times (a kernel of known "
@@ -239,6 +239,7 @@ class RealRun:
         self.hbm_peak = 0
 
     def hook(self, d, sb, plan, psi, tasks, step):
+        from tsc_tri_t1t2j1j2_amd import capi
         """on_plan hook of the DMRG: time H.psi once, on the first centre
         superblock of the mixed window (psi is its start vector)."""
         if self.hpsi is not None or not getattr(d, "_measure_next", False):
@@ -248,6 +249,13 @@ class RealRun:
         _, pm = tasks.flops()
         self.hpsi, self.roof, self.wall = hpsi_measure(self.ctx, plan, psi, self.args, self.prec, arrays, pm,
                                                        "yc6-m6000")
+        # the window's steps run with the default work units in both
+        # precisions: put this plan's back
+        scratch = self.ctx.vec(psi.n)
+        for p in {self.prec, capi.PREC_F64}:
+            if p != capi.PREC_F32:
+                plan.autotune(psi, scratch, p, target=2048.0)
+        scratch.close()
         self.hpsi.update({"psi_multiplet_dim": int(sb.super.dim), "sectors": int(sb.super.num),
                           "operator_arena_complex": int(step.len)})
         free, _ = self.ctx.mem_info()
@@ -288,9 +296,6 @@ class RealRun:
             # from the same start vectors (same-path error); the first step's
             # FP64 solve is the FP64 window's first step (same start vector)
             d.shadow, d.shadow_f64 = [], False
-            # work units of the window's plans: the choice the hook tuned for
-            # this plan shape (cached; no timing after the first)
-            d.autotune_flops = 1e12
             recs = d.centre_window(1 if args.no_window else k, tol, 6000, terr, tag=f"window-{name}",
                                    shadow_from=1 if p != capi.PREC_F64 else None)
             windows[name] = {"steps": recs, "shadow": list(d.shadow)}

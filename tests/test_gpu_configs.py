@@ -56,7 +56,8 @@ def test_yc4_4x6_m256_six_sweeps(ctx):
     # truncation errors are reported for every sweep step and stay small
     last = [h for h in f64.history if h[0] == "sweep6"]
     assert len(last) == 2 * (N - 3)
-    assert all(0.0 <= h[3] < 1e-4 for h in last)
+    # (a fully kept spectrum leaves a rounding remainder of either sign)
+    assert all(-1e-12 <= h[3] < 1e-4 for h in last)
     assert all(h[4] <= 257 for h in last)
     # variational order in the kept multiplets
     small = DMRG(ctx, 6, 4, kept_max=64, prec=capi.PREC_F64, **kw)
