@@ -60,6 +60,7 @@ def parse():
                                                            "(its first H.psi FP64)")
     ap.add_argument("--no-synthetic", action="store_true", help="skip the isolated 400x24 H.psi superblock")
     ap.add_argument("--no-window", action="store_true", help="real run: time H.psi only, no centre window")
+    ap.add_argument("--no-tune", action="store_true", help="time H.psi with the default work units (no autotuning)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -122,9 +123,12 @@ def hpsi_measure(ctx, plan, vin, args, prec, arrays, flops_pairmin, workload):
     vref = ctx.vec(vin.n)
     try:
         t_tune = time.perf_counter()
-        tuned = {"f64": plan.autotune(vin, vout, capi.PREC_F64)}
-        if prec != capi.PREC_F64:
-            tuned[args.prec] = plan.autotune(vin, vout, prec)
+        if args.no_tune:
+            tuned = {"f64": 2048.0, args.prec: 2048.0}
+        else:
+            tuned = {"f64": plan.autotune(vin, vout, capi.PREC_F64)}
+            if prec != capi.PREC_F64:
+                tuned[args.prec] = plan.autotune(vin, vout, prec)
         tuned["seconds"] = round(time.perf_counter() - t_tune, 2)
         w64, a64, b64 = time_gpu(ctx, plan, vin, vref, capi.PREC_F64, args.steps, args.warmup)
         print(f"# f64: {w64 * 1e3:.2f} ms/H.psi, {F / w64 / 1e12:.2f} TFLOP/s (stage 1 {a64:.2f} ms, stage 2 "

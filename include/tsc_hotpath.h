@@ -198,16 +198,20 @@ int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precision);
 int tsc_plan_last_times(const tsc_plan *p, float *ms_stage1, float *ms_stage2);
 
 /*
- * Autotuner of the work units: cuts the plan's units for `precision` (F64 or
- * BF16X3) with each candidate target of the unit builder (2048, 4096, 8192
- * units: how finely long contribution lists are split), times H.psi with
- * each on in -> out (out is overwritten), and keeps the fastest. The choice
- * is cached per (precision, log2 of the plan's flops, log2 of its output
- * blocks); a plan of a cached shape takes it without timing unless
- * flags & 1. *chosen = the target kept (0 for F32, whose units are fixed).
- * flags & 2: no timing, cut the units for the target given in *chosen.
- * Results agree with the untuned plan to rounding (the split changes the
- * order of partial sums). Has no counterpart in the reference.
+ * Autotuner of the work units and tile shape: cuts the plan's units for
+ * `precision` (F64 or BF16X3) with each candidate target of the unit
+ * builder (2048, 4096, 8192 units: how finely long contribution lists are
+ * split; FP64 also weighs its second tile shape, 64x64 grouped tiles
+ * instead of 128x64 LDS units, reported as target 1), times H.psi with each
+ * on in -> out (out is overwritten), and keeps the fastest. The choice is
+ * cached per (precision, log2 of the plan's flops, log2 of its output
+ * blocks, sector-size histogram: tenths of the stage-2 flops in output
+ * blocks whose larger side is < 64, < 128, < 256, < 512, >= 512); a plan of
+ * a cached shape takes it without timing unless flags & 1. *chosen = the
+ * target kept (0 for F32, whose units are fixed). flags & 2: no timing, cut
+ * the units for the target given in *chosen (1 selects the FP64 64x64
+ * tiles). Results agree with the untuned plan to rounding (the split
+ * changes the order of partial sums). Has no counterpart in the reference.
  */
 int tsc_plan_autotune(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precision, int flags, double *chosen);
 

@@ -161,3 +161,71 @@ def test_fortran_descriptor_types_match_the_c_structs(tmp_path):
         ctypes.sizeof(t) for t in (native.Sector, native.OpBlock, native.OpDesc, native.PairDesc, native.TermDesc,
                                    native.SbDesc, native.SiteOps, native.ModelDesc, native.Bond)]
     assert got == want, (got, want)
+
+
+class _RecordingLib:
+    """Stand-in for the library that records create/destroy calls (no GPU)."""
+
+    def __init__(self):
+        self.calls = []
+        self.next = 1
+
+    def _new(self, kind, out):
+        out._obj.value = self.next
+        self.calls.append(("create", kind, self.next))
+        self.next += 1
+        return 0
+
+    def tsc_ctx_create(self, device, out):
+        return self._new("ctx", out)
+
+    def tsc_vec_create(self, ctx, n, out):
+        return self._new("vec", out)
+
+    def tsc_plan_create(self, ctx, *args):
+        return self._new("plan", args[-1])
+
+    def tsc_ctx_destroy(self, h):
+        self.calls.append(("destroy", "ctx", h.value if hasattr(h, "value") else h))
+        return 0
+
+    def tsc_vec_destroy(self, h):
+        self.calls.append(("destroy", "vec", h.value if hasattr(h, "value") else h))
+        return 0
+
+    def tsc_plan_destroy(self, h):
+        self.calls.append(("destroy", "plan", h.value if hasattr(h, "value") else h))
+        return 0
+
+
+@pytest.mark.parametrize("order", ["ctx_first", "objects_first", "gc"])
+def test_teardown_never_frees_after_the_context(monkeypatch, order):
+    """Vectors and plans released after their context (the order Python may
+    finalise objects in at interpreter exit, or a garbage-collected cycle)
+    issue no device free: the library never sees a vector or plan destroy
+    after the destroy of its context."""
+    import gc
+    rec = _RecordingLib()
+    monkeypatch.setattr(capi, "_lib", rec)
+    ctx = capi.Context(0)
+    v = capi.Vec(ctx, 4)
+    z = np.zeros(0, capi.CONTRIB_DTYPE)
+    zo = np.zeros(0, capi.OUTBLOCK_DTYPE)
+    p = capi.Plan(ctx, np.zeros(1, np.complex128), 1, 0, z, zo, z, zo, 1, 1)
+    if order == "ctx_first":
+        ctx.close()
+        v.close()
+        p.close()
+    elif order == "objects_first":
+        p.close()
+        v.close()
+        ctx.close()
+    else:
+        ring = [ctx, v, p]
+        ring.append(ring)   # a cycle: finalisers run in the collector's order
+        del ctx, v, p, ring
+        gc.collect()
+    kinds = [(a, k) for (a, k, _) in rec.calls]
+    assert ("destroy", "ctx") in kinds
+    after = kinds[kinds.index(("destroy", "ctx")) + 1:]
+    assert not [c for c in after if c[0] == "destroy"], rec.calls

@@ -233,6 +233,41 @@ def test_two_stage_intermediates_in_both_roles(ctx, prec):
     assert _rel(out2, 2.0 * out) < TOL[prec] * 10
 
 
+@pytest.mark.parametrize("case", ["op_modes", "two_stage"])
+def test_fp64_tile_shapes_agree(ctx, case):
+    """The FP64 path's two tile shapes the autotuner chooses between (128x64
+    LDS units, 64x64 grouped tiles, selected with target 1) give the same
+    products for every operand mode and for intermediates in both roles."""
+    if case == "op_modes":
+        rng = np.random.default_rng(100)
+        ops, nops, vin, nin, c2, o2, nout, expected = _random_case(rng)
+        nmid, c1, o1 = 0, [], []
+        nops, nin = max(nops, 1), max(nin, 1)
+        x = np.zeros(nin, np.complex128)
+        x[:len(vin)] = vin[:nin]
+    else:
+        rng = np.random.default_rng(500)
+        ops, nops, vin, nin, nmid, c1, o1, c2, o2, nout, expected = _two_stage_case(rng)
+        x = vin
+    plan = capi.Plan(ctx, ops, nops, nmid, np.array(c1, capi.CONTRIB_DTYPE), np.array(o1, capi.OUTBLOCK_DTYPE),
+                     np.array(c2, capi.CONTRIB_DTYPE), np.array(o2, capi.OUTBLOCK_DTYPE), nin, nout)
+    ref = plan.apply_host(x, capi.PREC_F64)
+    vi, vo = ctx.vec_from(x), ctx.vec(nout)
+    assert plan.autotune(vi, vo, capi.PREC_F64, target=1.0) == 1.0
+    got = plan.apply_host(x, capi.PREC_F64)
+    assert _rel(got, ref) < 1e-13
+    for off, ldc, acc in expected:
+        m, n = acc.shape
+        g = got[off:off + ldc * n].reshape((ldc, n), order="F")[:m, :]
+        assert _rel(g, acc) < TOL[capi.PREC_F64] * 10
+    # back to the 128-row units
+    plan.autotune(vi, vo, capi.PREC_F64, target=2048.0)
+    assert _rel(plan.apply_host(x, capi.PREC_F64), ref) < 1e-13
+    vi.close()
+    vo.close()
+    plan.close()
+
+
 def test_plan_rejects_out_of_bounds_operand(ctx):
     e = np.zeros(1, capi.CONTRIB_DTYPE)
     e["a_off"], e["lda"], e["a_buf"], e["a_op"] = 0, 4, capi.BUF_OPS, capi.OP_N
@@ -410,7 +445,8 @@ def test_work_unit_targets_and_autotuner(ctx, prec):
     vout = ctx.vec(sb.psi_len)
     plan.apply_graph(vin, vout)             # a graph of the first units
     ctx.sync()
-    for target in (512.0, 2048.0, 4096.0, 8192.0, 32768.0):
+    targets = (512.0, 2048.0, 4096.0, 8192.0, 32768.0) + ((1.0, 2048.0) if prec == capi.PREC_F64 else ())
+    for target in targets:
         plan.autotune(vin, vout, prec, target=target)
         vout.fill_gaussian(3, 1, 1.0)
         plan.apply(vin, vout, prec)
@@ -422,7 +458,8 @@ def test_work_unit_targets_and_autotuner(ctx, prec):
         ctx.sync()
         assert _rel(vout.download(), eager) < 1e-15, target
     chosen = plan.autotune(vin, vout, prec, force=True)
-    assert chosen in (2048.0, 4096.0, 8192.0)
+    # FP64 also weighs its 64x64 tile shape (1)
+    assert chosen in ((2048.0, 4096.0, 8192.0, 1.0) if prec == capi.PREC_F64 else (2048.0, 4096.0, 8192.0))
     plan2 = sb.make_plan(ctx, sb.ops_host())
     assert plan2.autotune(vin, vout, prec) == chosen
     got = plan2.apply_host(psi, prec)

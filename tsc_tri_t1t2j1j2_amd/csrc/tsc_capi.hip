@@ -1,6 +1,7 @@
 // C-ABI of the hot path: contexts, grouped-contraction plans (H.psi),
 // and the Lanczos driver.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -400,6 +401,7 @@ extern "C" int tsc_plan_destroy(tsc_plan *p) {
     hipFree(p->d_r[s]);
   }
   hipFree(p->d_part32);
+  hipFree(p->d_part64);
   tsc::pk_free(p->pk);
   tsc::gd_free(p->gd);
   for (auto v : p->kry) hipFree(v);
@@ -553,6 +555,12 @@ int ensure_precision(tsc_plan *p, int prec) {
   }
   if (prec == TSC_PREC_F64) {
     if (!p->d_mid64 && p->mid_elems > 0) TSC_HIP_CHECK(hipMalloc((void **)&p->d_mid64, sizeof(double2) * p->mid_elems));
+    if (p->f64_kernel == 1) {
+      const int64_t slots64 = std::max(p->nslot[0], p->nslot[1]);
+      if (slots64 > 0 && !p->d_part64)
+        TSC_HIP_CHECK(hipMalloc(&p->d_part64, sizeof(double2) * (size_t)kTM * kTN * (size_t)slots64));
+      return TSC_OK;
+    }
     return gd_build(p);
   }
   const int64_t slots = std::max(p->nslot[0], p->nslot[1]);
@@ -572,14 +580,15 @@ int ensure_precision(tsc_plan *p, int prec) {
 int enqueue_hpsi(tsc_plan *p, const double2 *in, double2 *out, int prec, bool events) {
   hipStream_t s = p->ctx->stream;
   if (prec == TSC_PREC_BF16X3) return pk_enqueue(p, in, out, events);
-  if (prec == TSC_PREC_F64) return gd_enqueue(p, in, out, events);
+  if (prec == TSC_PREC_F64 && p->f64_kernel == 0) return gd_enqueue(p, in, out, events);
+  const bool f64 = prec == TSC_PREC_F64;
   Bufs b;
-  b.p[TSC_BUF_OPS] = p->d_ops32;
+  b.p[TSC_BUF_OPS] = f64 ? (const void *)p->d_ops64 : (const void *)p->d_ops32;
   b.p[TSC_BUF_IN] = in;
-  b.p[TSC_BUF_MID] = p->d_mid32;
+  b.p[TSC_BUF_MID] = f64 ? (const void *)p->d_mid64 : (const void *)p->d_mid32;
   b.p[TSC_BUF_OUT] = out;
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[0], s));
-  void *part = p->d_part32;
+  void *part = f64 ? p->d_part64 : p->d_part32;
   TSC_HIP_CHECK(launch_grouped(prec, b, p->d_o[0], p->d_c[0], p->d_t[0], p->nt[0], p->d_r[0], p->nr[0], part, s,
                                p->ctx->mfma_rowmode));
   if (events) TSC_HIP_CHECK(hipEventRecord(p->ev[1], s));
@@ -658,10 +667,34 @@ extern "C" int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int 
 }
 
 namespace {
-// tuned work-unit targets, shared by plans of the same coarse shape:
-// (precision, 2 log2 flops, 2 log2 stage-2 output blocks) -> make_tiles target
-std::map<std::tuple<int, int, int>, double> g_tune_cache;
+// tuned choices (work-unit target, or the FP64 tile shape), shared by plans
+// of the same shape: precision, 2 log2 flops, 2 log2 stage-2 output
+// blocks, and the sector-size histogram of the work, as the share (in
+// tenths) of the stage-2 flops in output blocks whose larger side is < 64,
+// < 128, < 256, < 512 and >= 512
+typedef std::array<int, 8> TuneKey;
+std::map<TuneKey, double> g_tune_cache;
 std::mutex g_tune_mu;
+
+TuneKey tune_key(const tsc_plan *p, int prec) {
+  double share[5] = {0, 0, 0, 0, 0}, tot = 0.0;
+  const std::vector<tsc_outblock> &o = p->h_o[1];
+  const std::vector<tsc_contrib> &c = p->h_c[1];
+  for (const tsc_outblock &ob : o) {
+    double f = 0.0;
+    for (int k = ob.cbeg; k < ob.cend; ++k) f += (double)ob.m * ob.n * std::max(1, c[k].k);
+    const int big = std::max(ob.m, ob.n);
+    const int cls = big < 64 ? 0 : big < 128 ? 1 : big < 256 ? 2 : big < 512 ? 3 : 4;
+    share[cls] += f;
+    tot += f;
+  }
+  TuneKey k;
+  k[0] = prec;
+  k[1] = (int)std::lround(2.0 * std::log2(std::max(p->flops, 1.0)));
+  k[2] = (int)std::lround(2.0 * std::log2(std::max<double>((double)o.size(), 1.0)));
+  for (int i = 0; i < 5; ++i) k[3 + i] = tot > 0.0 ? (int)std::lround(10.0 * share[i] / tot) : 0;
+  return k;
+}
 
 void drop_graph(tsc_plan *p, int prec) {
   if (p->gexec[prec]) hipGraphExecDestroy(p->gexec[prec]);
@@ -683,17 +716,27 @@ extern "C" int tsc_plan_autotune(tsc_plan *p, const tsc_vec *in, tsc_vec *out, i
     *chosen = 0.0;
     return TSC_OK;
   }
-  auto retile = [&](double d) { return prec == TSC_PREC_F64 ? gd_retile(p, d) : pk_retile(p, d); };
+  // a candidate: a work-unit target of the 128-row units (>= 64), or 1 =
+  // FP64 on the 64x64 grouped tiles (the other tile shape of that path)
+  auto retile = [&](double d) {
+    if (prec == TSC_PREC_F64) {
+      p->f64_kernel = d == 1.0 ? 1 : 0;
+      int r = ensure_precision(p, prec);
+      if (r || d == 1.0) return r;
+      return gd_retile(p, d);
+    }
+    return pk_retile(p, d);
+  };
   if (flags & 2) {
     // a given target, no timing
     const double d = *chosen;
-    TSC_REQUIRE(d >= 64.0 && d <= 1e7, "tsc_plan_autotune: work-unit target outside [64, 1e7]");
+    TSC_REQUIRE((d >= 64.0 && d <= 1e7) || (d == 1.0 && prec == TSC_PREC_F64),
+                "tsc_plan_autotune: work-unit target outside [64, 1e7] (or 1: FP64 64x64 tiles)");
     rc = retile(d);
     drop_graph(p, prec);
     return rc;
   }
-  const auto key = std::make_tuple(prec, (int)std::lround(2.0 * std::log2(std::max(p->flops, 1.0))),
-                                   (int)std::lround(2.0 * std::log2(std::max<double>((double)p->h_o[1].size(), 1.0))));
+  const auto key = tune_key(p, prec);
   if (!(flags & 1)) {
     double v = 0.0;
     {
@@ -709,7 +752,8 @@ extern "C" int tsc_plan_autotune(tsc_plan *p, const tsc_vec *in, tsc_vec *out, i
     }
   }
   // time each candidate: one H.psi to settle, then three on the plan's events
-  static const double cand[] = {2048.0, 4096.0, 8192.0};
+  std::vector<double> cand = {2048.0, 4096.0, 8192.0};
+  if (prec == TSC_PREC_F64) cand.push_back(1.0);
   hipStream_t s = p->ctx->stream;
   double best = cand[0], best_ms = 1e300;
   for (double d : cand) {

@@ -505,8 +505,18 @@ hipError_t launch_grouped(int prec, const Bufs &bufs, const tsc_outblock *d_bloc
                           hipStream_t stream, const int *modes) {
   if (ntiles <= 0) return hipSuccess;
   dim3 grid(ntiles), block(256);
-  // FP64 runs on the direct-to-LDS kernels (tsc_gd.hip), BF16x3 on the
-  // packed-operand ones (tsc_pk.hip)
+  // FP64 runs by default on the direct-to-LDS kernels (tsc_gd.hip; this
+  // 64x64 register-staged one is the tile shape the autotuner weighs against
+  // them), BF16x3 on the packed-operand ones (tsc_pk.hip)
+  if (prec == TSC_PREC_F64) {
+    hipLaunchKernelGGL((grouped_kernel<double, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
+                       modes[0], part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nreds <= 0) return e;
+    hipLaunchKernelGGL(reduce_kernel<double>, dim3(nreds), dim3(256), 0, stream, bufs, d_blocks, d_reds,
+                       (const void *)part);
+    return hipGetLastError();
+  }
   if (prec != TSC_PREC_F32) return hipErrorNotSupported;
   hipLaunchKernelGGL((grouped_kernel<float, 16>), grid, block, 0, stream, bufs, d_blocks, d_contribs, d_tiles,
                      modes[1], part);

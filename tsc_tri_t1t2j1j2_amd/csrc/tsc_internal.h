@@ -208,6 +208,8 @@ struct tsc_plan {
   int nr[2] = {0, 0};
   int64_t nslot[2] = {0, 0};           // partial-tile slots per stage
   void *d_part32 = nullptr;            // partial tiles, FP32 path
+  void *d_part64 = nullptr;            // partial tiles, FP64 64x64 path
+  int f64_kernel = 0;                  // FP64 kernel: 0 the 128x64 LDS units (tsc_gd.hip), 1 the 64x64 grouped tiles
   double flops = 0.0;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool timed = false;
