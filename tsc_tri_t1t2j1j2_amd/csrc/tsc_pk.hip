@@ -123,24 +123,19 @@ __device__ __forceinline__ double2 ld_op64(const double2 *X, int64_t off, int ld
 }
 
 __device__ __forceinline__ void scale_acc(f16v (&R)[2][2], f16v (&I)[2][2], float sr, float si) {
-  if (si == 0.f) {
+  // element by element, one temporary at a time: the accumulators fill most
+  // of the register budget (whole-vector temporaries spilled to scratch);
+  // with si == 0 the results equal x * sr exactly
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        R[a][b] *= sr;
-        I[a][b] *= sr;
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float x = R[a][b][r], y = I[a][b][r];
+        R[a][b][r] = x * sr - y * si;
+        I[a][b][r] = x * si + y * sr;
       }
-  } else {
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const f16v x = R[a][b], y = I[a][b];
-        R[a][b] = x * sr - y * si;
-        I[a][b] = x * si + y * sr;
-      }
-  }
 }
 
 
@@ -167,7 +162,9 @@ __global__ __launch_bounds__(256, 2) void pk_gemm_kernel(PkPtrs P, bf8 *__restri
                                                          const PkContrib *__restrict__ pcs,
                                                          const Tile *__restrict__ tiles, float2 *__restrict__ part) {
   __shared__ bf8 lds0[32][64], lds1[32][64];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, l32 = lane & 31, h = lane >> 5;
+  // the wave index is wave-uniform: per-wave conditions become scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, l32 = lane & 31, h = lane >> 5;
   const int wa = wave & 1, wb = wave >> 1;
   const Tile t = tiles[blockIdx.x];
   if (t.blk < 0) return;  // padding unit
