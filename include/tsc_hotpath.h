@@ -192,6 +192,26 @@ int tsc_hpsi_host(tsc_plan *p, const double *in_c128, double *out_c128, int prec
  * TSC_OP_CONJ operands are rejected with TSC_EINVAL.
  */
 int tsc_hpsi_loop(tsc_plan *p, const tsc_vec *in, tsc_vec *out);
+
+/*
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc.
+ * out = H in in FP64, as tsc_hpsi does, on the context stream (enqueued
+ * only; tsc_ctx_sync waits for it), with the plan's 64x64 grouped FP64
+ * tiles. in_c128 / out_c128 are device pointers to at least the plan's
+ * input / output lengths of complex128 values, and must differ. The stage-1
+ * intermediates and the partial tiles of split work units go to a static
+ * device buffer of TSC_NOALLOC_SCRATCH_BYTES, which each device the library
+ * runs on holds until the process ends, instead of plan-owned memory: the
+ * call allocates nothing. A plan whose intermediates and partial tiles
+ * (each part rounded up to 256 bytes) need more than that is rejected with
+ * TSC_EINVAL before anything is launched. Every call on one device shares
+ * the buffer: calls that overlap on one GPU (other streams, threads or
+ * contexts) race.
+ */
+#define TSC_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int tsc_hpsi_noalloc(tsc_plan *p, const double *in_c128, double *out_c128);
 /* capture one application in a hipGraph and replay it from then on */
 int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int precision);
 /* device time (ms) of the last stage-1 / stage-2 launches, HIP events */

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libtsc_hotpath.so")
 
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
 PREC_NAMES = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
+# size of the static device buffer of tsc_hpsi_noalloc (TSC_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 OP_N, OP_T, OP_C, OP_CONJ, OP_I = 0, 1, 2, 3, 4
 BUF_OPS, BUF_IN, BUF_MID, BUF_OUT = 0, 1, 2, 3
 
@@ -40,7 +42,8 @@ EXPORTS = [
     "tsc_vec_norm", "tsc_vec_dptr", "tsc_vec_fill_gaussian", "tsc_vec_axpby", "tsc_vec_copy",
     "tsc_vec_copy_range", "tsc_vec_upload_range", "tsc_vec_download_range",
     "tsc_plan_create", "tsc_plan_create_dev", "tsc_plan_destroy", "tsc_plan_update_ops", "tsc_plan_fill_ops_gaussian",
-    "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_loop", "tsc_hpsi_graph", "tsc_plan_last_times",
+    "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_loop", "tsc_hpsi_noalloc", "tsc_hpsi_graph",
+    "tsc_plan_last_times",
     "tsc_plan_autotune",
     "tsc_lanczos", "tsc_lanczos_replays", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_select_states",
     "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_flops", "tsc_tasks_copy",
@@ -106,6 +109,7 @@ def lib():
         "tsc_hpsi_host": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
         "tsc_hpsi_graph": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
         "tsc_hpsi_loop": (c_int, [c_void_p, c_void_p, c_void_p]),
+        "tsc_hpsi_noalloc": (c_int, [c_void_p, c_void_p, c_void_p]),
         "tsc_plan_last_times": (c_int, [c_void_p, P(c_float), P(c_float)]),
         "tsc_plan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, P(c_double)]),
         "tsc_lanczos": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_int, c_int, P(c_double), P(c_int32)]),
@@ -221,6 +225,10 @@ class Vec:
         check(lib().tsc_vec_download(self.h, _ptr(out)))
         return out
 
+    def dptr(self):
+        """Device address of the vector's data (complex128)."""
+        return lib().tsc_vec_dptr(self.h)
+
     def dot_re(self, other):
         r = c_double()
         check(lib().tsc_vec_dot_re(self.h, other.h, ctypes.byref(r)))
@@ -326,6 +334,12 @@ class Plan:
         """FP64 H.psi as one ZGEMM call per contribution (the per-product
         baseline; plans of GEMM contributions only)."""
         check(lib().tsc_hpsi_loop(self.h, vin.h, vout.h))
+
+    def apply_noalloc(self, vin, vout):
+        """FP64 H.psi that allocates nothing: intermediates and partial tiles
+        in the library's static scratch buffer (NOALLOC_SCRATCH_BYTES per
+        device); enqueued on the context stream."""
+        check(lib().tsc_hpsi_noalloc(self.h, vin.dptr(), vout.dptr()))
 
     def apply_graph(self, vin, vout, prec=PREC_F64):
         check(lib().tsc_hpsi_graph(self.h, vin.h, vout.h, int(prec)))

@@ -128,6 +128,41 @@ void tridiag_vector(const double *d, const double *e, int n, double lam, double 
 
 using namespace tsc;
 
+// ---------------------------------------------------------------------------
+// Scratch memory of tsc_hpsi_noalloc: one static device buffer, which every
+// device the library runs on holds until the process ends. On the host the
+// array's name is only a stand-in; its address on a device is looked up once
+// per device, with that device current, when the first context is created
+// on it, and host code uses that address alone (kernel arguments, copies).
+static __device__ __align__(256) unsigned char scratch[TSC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices] = {};
+
+// the device of this call is current
+int scratch_setup(int device) {
+  TSC_REQUIRE(device >= 0 && device < kScratchDevices, "device index beyond the scratch address table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return TSC_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    return TSC_EHIP;
+  }
+  g_scratch_ptr[device] = ptr;
+  return TSC_OK;
+}
+
+void *scratch_address(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 
 extern "C" const char *tsc_last_error(void) { return g_err.c_str(); }
 extern "C" int tsc_version(void) { return 1; }
@@ -154,6 +189,10 @@ extern "C" int tsc_ctx_create(int device, tsc_ctx **out) {
   TSC_HIP_CHECK(hipGetDeviceCount(&ndev));
   TSC_REQUIRE(device >= 0 && device < ndev, "tsc_ctx_create: no such device");
   TSC_HIP_CHECK(hipSetDevice(device));
+  {
+    const int rc = scratch_setup(device);
+    if (rc) return rc;
+  }
   tsc_ctx *c = new tsc_ctx();
   c->device = device;
   TSC_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -630,6 +669,42 @@ extern "C" int tsc_hpsi_host(tsc_plan *p, const double *in, double *out, int pre
   tsc_vec_destroy(vi);
   tsc_vec_destroy(vo);
   return rc;
+}
+
+// FP64 H.psi on the plan's 64x64 grouped tiles with the intermediates and
+// the partial tiles in the static scratch buffer: [intermediates | partial
+// tiles], each part starting on a 256-byte boundary. Nothing there has to
+// start at zero: the stage-1 blocks write every intermediate window in full
+// before stage 2 reads it, and each split unit writes every in-block element
+// of its partial tile before the reduction reads it (the plan-owned buffers
+// of tsc_hpsi are uninitialised device memory too). No allocation, no
+// library call that allocates, no synchronisation.
+extern "C" int tsc_hpsi_noalloc(tsc_plan *p, const double *in_c128, double *out_c128) {
+  TSC_REQUIRE(p && p->ctx, "tsc_hpsi_noalloc: null plan");
+  TSC_REQUIRE((in_c128 || p->in_elems == 0) && (out_c128 || p->out_elems == 0), "tsc_hpsi_noalloc: null vector");
+  TSC_REQUIRE(in_c128 != out_c128 || p->in_elems == 0 || p->out_elems == 0, "tsc_hpsi_noalloc: in and out must differ");
+  void *base = scratch_address(p->ctx->device);
+  TSC_REQUIRE(base, "tsc_hpsi_noalloc: no scratch address for the plan's device");
+  const int64_t mid_bytes = (p->mid_elems * (int64_t)sizeof(double2) + 255) & ~(int64_t)255;
+  const int64_t slots = std::max(p->nslot[0], p->nslot[1]);
+  const int64_t part_bytes = slots * (int64_t)kTM * kTN * (int64_t)sizeof(double2);
+  if (mid_bytes + part_bytes > (int64_t)TSC_NOALLOC_SCRATCH_BYTES) {
+    set_error("invalid: tsc_hpsi_noalloc: the plan needs " + std::to_string(mid_bytes) +
+              " bytes of intermediates and " + std::to_string(part_bytes) + " bytes of partial tiles; the scratch "
+              "buffer holds " + std::to_string((int64_t)TSC_NOALLOC_SCRATCH_BYTES) + " bytes");
+    return TSC_EINVAL;
+  }
+  Bufs b;
+  b.p[TSC_BUF_OPS] = p->d_ops64;
+  b.p[TSC_BUF_IN] = in_c128;
+  b.p[TSC_BUF_MID] = base;
+  b.p[TSC_BUF_OUT] = out_c128;
+  void *part = static_cast<unsigned char *>(base) + mid_bytes;
+  hipStream_t s = p->ctx->stream;
+  for (int g = 0; g < 2; ++g)
+    TSC_HIP_CHECK(launch_grouped(TSC_PREC_F64, b, p->d_o[g], p->d_c[g], p->d_t[g], p->nt[g], p->d_r[g], p->nr[g],
+                                 part, s, p->ctx->mfma_rowmode));
+  return TSC_OK;
 }
 
 extern "C" int tsc_hpsi_graph(tsc_plan *p, const tsc_vec *in, tsc_vec *out, int prec) {

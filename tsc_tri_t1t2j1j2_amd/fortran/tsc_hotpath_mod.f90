@@ -268,6 +268,11 @@ module tsc_hotpath
       import :: c_int, c_ptr
       type(c_ptr), value :: plan, vin, vout
     end function
+    ! in_c128 / out_c128: device addresses (tsc_vec_dptr); allocates nothing
+    integer(c_int) function tsc_hpsi_noalloc(plan, in_c128, out_c128) bind(c, name="tsc_hpsi_noalloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: plan, in_c128, out_c128
+    end function
     integer(c_int) function tsc_hpsi_graph(plan, vin, vout, prec) bind(c, name="tsc_hpsi_graph")
       import :: c_int, c_ptr
       type(c_ptr), value :: plan, vin, vout
