@@ -163,6 +163,25 @@ def test_fortran_descriptor_types_match_the_c_structs(tmp_path):
     assert got == want, (got, want)
 
 
+def test_work_units_cover_every_tile_once_and_spread_over_the_xcds(tmp_path):
+    """The library's work-unit builder, compiled into a host program: every
+    tile of every output block gets units whose contribution ranges tile the
+    block's range exactly once, and the per-XCD work stays balanced, also
+    for one block much larger than an XCD's share (tests/native)."""
+    import shutil
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("no hipcc")
+    src = os.path.join(ROOT, "tests", "native", "work_units_check.cpp")
+    pkg = os.path.dirname(capi.LIB_PATH)
+    exe = tmp_path / "work_units_check"
+    subprocess.run([hipcc, "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I",
+                    os.path.join(pkg, "csrc"), src, "-o", str(exe), "-L", pkg, "-ltsc_hotpath",
+                    "-Wl,-rpath," + pkg], check=True, capture_output=True, timeout=300)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout + out.stderr
+
+
 class _RecordingLib:
     """Stand-in for the library that records create/destroy calls (no GPU)."""
 

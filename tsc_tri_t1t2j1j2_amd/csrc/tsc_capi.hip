@@ -309,10 +309,12 @@ namespace tsc {
 // windows, and output blocks whose smallest B operand window is the same
 // (blocks of one environment window: they read the same operator blocks)
 // do too; such blocks form one group that goes to a single XCD, units of
-// the same contribution chunk next to each other. Groups (capped at a
-// quarter of an XCD's share of the work) are dealt to the XCDs largest
-// first onto the least-loaded one, and the per-XCD lists are interleaved
-// (padded with empty units, blk = -1, that exit at once).
+// the same contribution chunk next to each other. A block costing more
+// than half an XCD's share is cut into near-square rectangles of tiles that
+// form groups of their own. Groups (capped at a quarter of an XCD's share
+// of the work) are dealt to the XCDs largest first onto the least-loaded
+// one, and the per-XCD lists are interleaved (padded with empty units,
+// blk = -1, that exit at once).
 constexpr int kXcds = 8;  // MI355X: 256 CUs in 8 XCDs
 
 Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm, int tn, double desired,
@@ -377,20 +379,47 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm
                                   nch});
       u.nslot += (int64_t)nti * ntj * nch;
     }
+    auto add_units = [&](Group &g, int ti0, int ti1, int tj0, int tj1) {
+      for (int ch = 0; ch < nch; ++ch)
+        for (int ti = ti0; ti < ti1; ++ti)
+          for (int tj = tj0; tj < tj1; ++tj) {
+            const int32_t slot = split ? (int32_t)(slot_base + (int64_t)(ti * ntj + tj) * nch + ch) : -1;
+            g.units.push_back(Tile{b, (int16_t)ti, (int16_t)tj, cut[ch], cut[ch + 1], slot, 0});
+            g.cost += ccut[ch];
+          }
+    };
+    double bcost = 0.0;
+    for (double x : ccut) bcost += x;
+    bcost *= (double)nti * ntj;
+    if (bcost > 2.0 * group_cap && (int64_t)nti * ntj > 1) {
+      // a block larger than half an XCD's share, which the dealing below could
+      // not balance: near-square rectangles of tiles of at most a group's
+      // cost, each a group of its own (one XCD reads the rectangle's A rows
+      // and B columns; smaller blocks stay whole, their operands read by one
+      // XCD only)
+      const int64_t want = std::min<int64_t>((int64_t)nti * ntj, (int64_t)std::ceil(bcost / group_cap));
+      int pr = (int)std::lround(std::sqrt((double)want * nti / ntj));
+      pr = std::max(1, std::min(nti, pr));
+      int pc = (int)std::min<int64_t>(ntj, (want + pr - 1) / pr);
+      pc = std::max(1, pc);
+      if ((int64_t)pr * pc < want) pr = (int)std::min<int64_t>(nti, (want + pc - 1) / pc);
+      for (int r = 0; r < pr; ++r)
+        for (int q = 0; q < pc; ++q) {
+          const int ti0 = (int)((int64_t)r * nti / pr), ti1 = (int)((int64_t)(r + 1) * nti / pr);
+          const int tj0 = (int)((int64_t)q * ntj / pc), tj1 = (int)((int64_t)(q + 1) * ntj / pc);
+          if (ti0 == ti1 || tj0 == tj1) continue;
+          groups.emplace_back();
+          add_units(groups.back(), ti0, ti1, tj0, tj1);
+        }
+      continue;
+    }
     auto it = open_group.find(key);
     if (it == open_group.end() || groups[it->second].cost >= group_cap) {
       groups.emplace_back();
       open_group[key] = groups.size() - 1;
       it = open_group.find(key);
     }
-    Group &g = groups[it->second];
-    for (int ch = 0; ch < nch; ++ch)
-      for (int ti = 0; ti < nti; ++ti)
-        for (int tj = 0; tj < ntj; ++tj) {
-          const int32_t slot = split ? (int32_t)(slot_base + (int64_t)(ti * ntj + tj) * nch + ch) : -1;
-          g.units.push_back(Tile{b, (int16_t)ti, (int16_t)tj, cut[ch], cut[ch + 1], slot, 0});
-          g.cost += ccut[ch];
-        }
+    add_units(groups[it->second], 0, nti, 0, ntj);
   }
   std::stable_sort(groups.begin(), groups.end(), [](const Group &x, const Group &y) { return x.cost > y.cost; });
   std::vector<std::vector<Tile>> lists(kXcds);
