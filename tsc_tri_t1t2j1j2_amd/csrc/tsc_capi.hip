@@ -333,8 +333,37 @@ Units make_tiles(const tsc_contrib *c, const tsc_outblock *o, int32_t no, int tm
     total += std::max(w, 1.0) * nt;
     ntile += nt;
   }
-  // aim at several units per workgroup slot of the device
-  const double target = (ntile >= (int64_t)desired) ? 1e300 : std::max(total / desired, 64.0);
+  // Cost (k) at which a tile's contribution list is cut into units: about
+  // `desired` units for plans with fewer tiles, and in any case no unit
+  // longer than kUnitCostCap, because the units of one output window that
+  // are resident together drift apart over a long list and stop sharing the
+  // XCD's L2. The partial tiles of cut units are capped at kMaxSlots: beyond
+  // that the cut cost grows.
+  constexpr double kUnitCostCap = 8192.0;
+  constexpr int64_t kMaxSlots = 16384;
+  auto slots_for = [&](double cut_cost) {
+    int64_t slots = 0;
+    for (int32_t b = 0; b < no; ++b) {
+      if (skip(b) || o[b].cend - o[b].cbeg <= 1) continue;
+      double w = 0.0;
+      for (int32_t ci = o[b].cbeg; ci < o[b].cend; ++ci) w += ccost(ci);
+      if (w <= cut_cost) continue;
+      int64_t nch = 0;
+      double acc = 0.0;
+      for (int32_t ci = o[b].cbeg; ci < o[b].cend; ++ci) {
+        acc += ccost(ci);
+        if (acc >= cut_cost || ci + 1 == o[b].cend) {
+          ++nch;
+          acc = 0.0;
+        }
+      }
+      slots += nch * (int64_t)((o[b].m + tm - 1) / tm) * ((o[b].n + tn - 1) / tn);
+    }
+    return slots;
+  };
+  double target = (ntile >= (int64_t)desired) ? kUnitCostCap
+                                              : std::min(std::max(total / desired, 64.0), kUnitCostCap);
+  for (int it = 0; it < 80 && slots_for(target) > kMaxSlots; ++it) target *= 1.25;
   Units u;
   struct Group {
     double cost = 0.0;
