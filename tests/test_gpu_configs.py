@@ -230,3 +230,40 @@ def test_yc6_m6000_centre_superblock(ctx):
         step.close()
         for m in blocks:
             m.arena.close()
+
+
+@pytest.mark.timeout(900)
+def test_yc6_m12000_centre_superblock(ctx):
+    """The YC6 centre superblock with every sector of the real M=6000 centre
+    blocks doubled (M=12000; psi 120,038,988 multiplet states in 25 sectors,
+    an operator arena of 3.9e9 complex values) fits one MI355X with FP64 and
+    BF16x3 state together: H is Hermitian (<phi|H psi> = <H phi|psi>*) and
+    BF16x3 stays within its error bound."""
+    sb, blocks, step, tasks, plan = _hist_superblock(ctx, 2.0)
+    n = sb.super.dim
+    assert n == 120038988 and sb.super.num == 25
+    vs = [ctx.vec(n) for _ in range(4)]
+    try:
+        psi, phi, hpsi, hphi = vs
+        psi.fill_gaussian(12, 1, 1.0)
+        phi.fill_gaussian(12, 2, 1.0)
+        plan.apply(psi, hpsi, capi.PREC_F64)
+        plan.apply(phi, hphi, capi.PREC_F64)
+        x, y, hx, hy = (v.download() for v in vs)
+        a, b = np.vdot(y, hx), np.vdot(hy, x)
+        print(f"<phi|H psi> = {a:.12e}, <H phi|psi> = {b:.12e}")
+        assert abs(a - b) <= 1e-12 * np.linalg.norm(hx) * np.linalg.norm(y)
+        del y, hy
+        plan.apply(psi, hphi, capi.PREC_BF16X3)
+        e = np.linalg.norm(hphi.download() - hx) / np.linalg.norm(hx)
+        free, total = ctx.mem_info()
+        print(f"BF16x3 rel err vs FP64 at YC6 M=12000: {e:.2e}; HBM in use {(total - free) / 2 ** 30:.1f} GiB")
+        assert e <= 3e-5
+    finally:
+        for v in vs:
+            v.close()
+        plan.close()
+        tasks.close()
+        step.close()
+        for m in blocks:
+            m.arena.close()
