@@ -23,9 +23,13 @@ the device.
 Normalisation (physical operators, no SU(2) factors left):
   n_i;   S_i.S_j = -sqrt3 [S_i x S_j]^0;   n_i n_j = [n_i x n_j]^0;
   (sum_s c+_is c_js + h.c.) / 2 = -([c+_i x c~_j]^0 + [c~_i x c+_j]^0) / sqrt2
-with i before j in the fermion order A, s1, B, s2. For the real
-Hamiltonian the last one is the value the reference writes as the real part
-of elec_cor (its imaginary part vanishes for a real ground state).
+with i before j in the fermion order A, s1, B, s2. These are the values the
+reference's files hold: its coupling rule (block.f90:376-635) with its site
+elements and Get_density_cor's factor sqrt(1+down_dif), negated for
+down_dif = 2 (measure.f90:2586-2587), gives <S_i.S_j>, <n_i n_j> and
+<sum_s c+_is c_js>, and its pair values (measure.f90:2762, 2806) are
+<D+_ab D_cd>. For the real Hamiltonian the hopping value is real; the
+imaginary columns the reference writes are rounding noise, written as 0.
 """
 import time
 from math import sqrt
