@@ -23,4 +23,9 @@ parameters):
                       energy; sum of <n_i> = N_e; sum of <S_i.S_j> = 0 for a
                       singlet; pair correlators equal overlaps of annihilated
                       states; the dimer singlet gives <D+ D> = 1
+  ref_measure.*       the reference's coupling rule and file factors as
+                      Fock-space matrices: on-site identities (S.S = 3/4 n,
+                      n n = n, sum_s c+_s c_s = n), two-site operators equal
+                      the physical ones; the standard Clebsch-Gordan phase in
+                      its place flips the pair operator (the check fails)
 """
