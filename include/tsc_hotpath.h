@@ -305,6 +305,16 @@ int tsc_rho_truncate(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, const int32
                      double *u_out_c128, int64_t u_cap, double *scalars_out);
 
 /*
+ * Wall milliseconds of the last tsc_rho_truncate / tsc_truncation_operator
+ * on this context: [0] rho (device allocation and build, or upload),
+ * [1] per-sector eigensolves, [2] eigenvalues to the host and the kept-state
+ * selection, [3] kept eigenvectors to the host. A measurement hook of
+ * Get_truncation_operator's stages (den_mat.f90:229-398); no counterpart in
+ * the reference.
+ */
+int tsc_rho_last_times(const tsc_ctx *ctx, float *ms4);
+
+/*
  * The kept-state selection of Get_truncation_operator alone
  * (den_mat.f90:255-364), host code that needs no device: from the
  * eigenvalues of every sector (ascending per sector, laid end to end) it

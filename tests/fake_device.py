@@ -132,4 +132,5 @@ def fake_rho_truncate(ctx, psi_vec, sd, ed, twos, side, kmin, kmax, terr):
 def install(monkeypatch):
     monkeypatch.setattr(capi, "Plan", FakePlan)
     monkeypatch.setattr(capi, "rho_truncate", fake_rho_truncate)
+    monkeypatch.setattr(capi, "rho_last_times", lambda ctx: [0.0, 0.0, 0.0, 0.0])
     return FakeCtx()

@@ -45,7 +45,7 @@ EXPORTS = [
     "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_loop", "tsc_hpsi_noalloc", "tsc_hpsi_graph",
     "tsc_plan_last_times",
     "tsc_plan_autotune",
-    "tsc_lanczos", "tsc_lanczos_replays", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_select_states",
+    "tsc_lanczos", "tsc_lanczos_replays", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_rho_last_times", "tsc_select_states",
     "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_flops", "tsc_tasks_copy",
     "tsc_model_tasks", "tsc_renorm_tasks", "tsc_tasks_layout", "tsc_tasks_lengths", "tsc_tasks_copy_arena",
     "tsc_tasks_plan", "tsc_tasks_plan_dev", "tsc_tasks_destroy",
@@ -117,6 +117,7 @@ def lib():
         "tsc_density_matrix": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int, c_void_p]),
         "tsc_truncation_operator": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
                                             c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+        "tsc_rho_last_times": (c_int, [c_void_p, P(c_float)]),
         "tsc_rho_truncate": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int, c_int32,
                                      c_int32, c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
         "tsc_select_states": (c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_double, c_void_p,
@@ -434,6 +435,14 @@ def rho_truncate(ctx, psi_vec, sys_dim, env_dim, twos, side, kept_min, kept_max,
                                  u.size, _ptr(scal)))
     return dict(eigval=eig, kept=kept, U=_split_u(dim, kept, u), num_state=int(scal[0]), trun_err=scal[1],
                 entropy=scal[2], entropy1=scal[3])
+
+
+def rho_last_times(ctx):
+    """Wall ms of the last rho / eigensolve / truncation call on ctx: rho
+    build, eigensolves, eigenvalues to the host + selection, U to the host."""
+    ms = (c_float * 4)()
+    check(lib().tsc_rho_last_times(ctx.h, ms))
+    return [float(x) for x in ms]
 
 
 def select_states(dim, twos, eigval, kept_min, kept_max, trun_err):

@@ -184,6 +184,10 @@ struct tsc_ctx {
   double *d_coef = nullptr;  // combination coefficients
   void **h_ptrs = nullptr;   // pinned staging
   double *h_coef = nullptr;
+  // wall milliseconds of the last tsc_rho_truncate / tsc_truncation_operator:
+  // rho (allocation, upload or build), eigensolves, eigenvalues to the host
+  // and kept-state selection, kept eigenvectors to the host
+  float rho_ms[4] = {0.f, 0.f, 0.f, 0.f};
 };
 
 struct tsc_vec {

@@ -8,6 +8,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -387,9 +388,14 @@ void select_states(int32_t nsec, const int32_t *dim, const int32_t *twos, const 
   scalars[3] = entropy1;
 }
 
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_t *twos, double2 *d_rho,
                     const std::vector<int64_t> &rho_off, int32_t kept_min, int32_t kept_max, double trun_err,
                     double *eigval_out, int32_t *kept_out, double *u_out, int64_t u_cap, double *scalars_out) {
+  auto t0 = std::chrono::steady_clock::now();
   std::vector<int64_t> w_off(nsec);
   int64_t wt = 0;
   for (int32_t s = 0; s < nsec; ++s) {
@@ -400,6 +406,8 @@ int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_
   TSC_HIP_CHECK(hipMalloc(&dw.p, sizeof(double) * std::max<int64_t>(wt, 1)));
   int rc = eig_sectors(ctx, nsec, dim, d_rho, rho_off, (double *)dw.p, w_off);
   if (rc) return rc;
+  ctx->rho_ms[1] = (float)ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
   std::vector<double> w(std::max<int64_t>(wt, 1));
   TSC_HIP_CHECK(hipMemcpy(w.data(), dw.p, sizeof(double) * wt, hipMemcpyDeviceToHost));
   select_states(nsec, dim, twos, w.data(), kept_min, kept_max, trun_err, kept_out, scalars_out);
@@ -411,6 +419,8 @@ int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_
               std::to_string(u_need) + " (sum over sectors of dim * kept; sum dim^2 always suffices)");
     return TSC_EINVAL;
   }
+  ctx->rho_ms[2] = (float)ms_since(t0);
+  t0 = std::chrono::steady_clock::now();
   // U_s = last kept_s eigenvector columns of sector s (den_mat.f90:392-398)
   int64_t uoff = 0;
   for (int32_t s = 0; s < nsec; ++s) {
@@ -427,6 +437,7 @@ int truncate_device(tsc_ctx *ctx, int32_t nsec, const int32_t *dim, const int32_
       set_error("density-matrix eigensolver returned NaN eigenvectors");
       return TSC_EHIP;
     }
+  ctx->rho_ms[3] = (float)ms_since(t0);
   return TSC_OK;
 }
 
@@ -464,9 +475,11 @@ extern "C" int tsc_truncation_operator(tsc_ctx *ctx, int32_t nsec, const int32_t
     off[s] = rho_elems;
     rho_elems += (int64_t)dim[s] * dim[s];
   }
+  auto t0 = std::chrono::steady_clock::now();
   DevBuf drho;
   TSC_HIP_CHECK(hipMalloc(&drho.p, sizeof(double2) * std::max<int64_t>(rho_elems, 1)));
   TSC_HIP_CHECK(hipMemcpy(drho.p, rho, sizeof(double2) * rho_elems, hipMemcpyHostToDevice));
+  ctx->rho_ms[0] = (float)ms_since(t0);
   return truncate_device(ctx, nsec, dim, twos, (double2 *)drho.p, off, kept_min, kept_max, trun_err, eigval_out,
                          kept_out, u_out, u_cap, scalars_out);
 }
@@ -485,13 +498,21 @@ extern "C" int tsc_rho_truncate(tsc_ctx *ctx, const tsc_vec *psi, int32_t nsec, 
     dim[s] = side == 0 ? sys_dim[s] : env_dim[s];
     rho_elems += (int64_t)dim[s] * dim[s];
   }
+  auto t0 = std::chrono::steady_clock::now();
   DevBuf rho;
   TSC_HIP_CHECK(hipMalloc(&rho.p, sizeof(double2) * std::max<int64_t>(rho_elems, 1)));
   std::vector<int64_t> off;
   int rc = build_rho(ctx, psi->d, psi->n, nsec, sys_dim, env_dim, side, (double2 *)rho.p, rho_elems, off);
   if (rc) return rc;
+  ctx->rho_ms[0] = (float)ms_since(t0);
   return truncate_device(ctx, nsec, dim.data(), twos, (double2 *)rho.p, off, kept_min, kept_max, trun_err,
                          eigval_out, kept_out, u_out, u_cap, scalars_out);
+}
+
+extern "C" int tsc_rho_last_times(const tsc_ctx *ctx, float *ms4) {
+  TSC_REQUIRE(ctx && ms4, "tsc_rho_last_times arguments");
+  for (int i = 0; i < 4; ++i) ms4[i] = ctx->rho_ms[i];
+  return TSC_OK;
 }
 
 extern "C" int tsc_select_states(int32_t nsec, const int32_t *dim, const int32_t *twos, const double *eigval,

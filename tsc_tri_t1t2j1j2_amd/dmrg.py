@@ -70,7 +70,7 @@ class DMRG:
                  kept_min=110, kept_max=300, kept_maxf=None, sweeps=6, lanzero=1e-6, trun_err=1e-15,
                  prec=capi.PREC_F64, refine=None, seed=1234, log=None, native_planner=True, resident=True,
                  lanczos_budget=(100, 4), shadow_f64=False, wave_guess=True, guessed_f64=False,
-                 autotune_flops=None, retain=None, warmup_ramp=None, on_plan=None):
+                 autotune_flops=None, retain=None, warmup_ramp=None, on_plan=None, renorm_prec=capi.PREC_F64):
         self.ctx = ctx
         self.Lx, self.Ny = Lx, Ny
         self.N = Lx * Ny
@@ -103,6 +103,11 @@ class DMRG:
         # iterations, which the FP64 refinement would repeat anyway)
         self.guessed_f64 = guessed_f64
         self._prec_now = prec
+        # renorm_prec: precision of the enlargement + U^H O U plan of the
+        # resident path (update_trun_*, Get_Hamiltonian_Sys/Env); FP64 by
+        # default, since the renormalised operators enter every later step
+        self.renorm_prec = renorm_prec
+        self.rho_parts = {}
         # superblocks with at least this many flops per H.psi get their work
         # units tuned (tsc_plan_autotune; the choice is cached per plan shape);
         # None: off (along a sweep the plan shapes change from step to step,
@@ -187,6 +192,10 @@ class DMRG:
         t0 = time.perf_counter()
         res = capi.rho_truncate(self.ctx, psi_vec, sd, ed, twos, side, kept_min, kept_max, trun_err)
         self.timers["truncate"] += time.perf_counter() - t0
+        # the library's own split of that call: rho build, eigensolves,
+        # selection, kept eigenvectors to the host
+        for name, ms in zip(("rho_build", "rho_eig", "rho_select", "rho_u"), capi.rho_last_times(self.ctx)):
+            self.rho_parts[name] = self.rho_parts.get(name, 0.0) + ms / 1e3
         dd = np.asarray(sd if side == 0 else ed, np.float64)
         kk = np.asarray(res["kept"], np.float64)
         self.trunc_bytes += 16.0 * (psi_vec.n + 3.0 * float(np.sum(dd * dd)) + float(np.sum(dd * kk)))
@@ -461,7 +470,7 @@ class DMRG:
             t0 = time.perf_counter()
             rp = ham.renormalize_plan(blk, site, to_new, keep, U, order, pairs=pairs, resolver=step.offset,
                                       prepared=prepared[side])
-            out[side] = made[side] = resident.renormalize(ctx, step, rp)
+            out[side] = made[side] = resident.renormalize(ctx, step, rp, self.renorm_prec)
             fl, ms = getattr(out[side], "renorm_stats", (0.0, 0.0))
             self.renorm_device[0] += fl
             self.renorm_device[1] += ms
@@ -573,6 +582,7 @@ class DMRG:
                 if shadow_from is not None:
                     self.shadow_f64 = l - l0 >= shadow_from
                 t_before = dict(self.timers)
+                r_before = dict(self.rho_parts)
                 guess = self._guess("LR", prev, self.sys_trun.get(l), self.env_trun.get(N - l - 1))
                 sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
                 r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err,
@@ -587,9 +597,10 @@ class DMRG:
                 prev = (r["psi"], r["bases"])
                 parts = {k: round(v - t_before.get(k, 0.0), 3) for k, v in self.timers.items()
                          if v - t_before.get(k, 0.0) > 0.0}
+                truncate_s = {k: round(v - r_before.get(k, 0.0), 4) for k, v in self.rho_parts.items()}
                 recs.append({"l": l, "E": r["E"], "seconds": r["seconds"], "matvecs": r["matvecs"],
                              "dim": r["dim"], "nsec": r["nsec"], "kept": r["sys_kept"], "trun_err": r["sys_err"],
-                             "parts_s": parts})
+                             "parts_s": parts, "truncate_parts_s": truncate_s})
                 self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
                 self._say(f"{tag} l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
                           f"dim={r['dim']} nsec={r['nsec']} mv={r['matvecs']} {r['seconds']:.2f}s")

@@ -283,6 +283,11 @@ module tsc_hotpath
       type(c_ptr), value :: plan
       real(c_float) :: ms1, ms2
     end function
+    integer(c_int) function tsc_rho_last_times(ctx, ms4) bind(c, name="tsc_rho_last_times")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float) :: ms4(4)
+    end function
     integer(c_int) function tsc_plan_autotune(plan, vin, vout, prec, flags, chosen) bind(c, name="tsc_plan_autotune")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: plan, vin, vout

@@ -31,6 +31,9 @@ def main(argv=None):
     ap.add_argument("--trun-err", type=float, default=1e-15)
     ap.add_argument("--prec", default="f64", choices=["f64", "f32", "bf16x3"])
     ap.add_argument("--refine", type=int, default=2)
+    ap.add_argument("--renorm-prec", default="f64", choices=["f64", "f32", "bf16x3"],
+                    help="precision of the enlargement + U^H O U plans (default f64: the renormalised operators "
+                         "enter every later step)")
     ap.add_argument("--no-guess", action="store_true", help="start every Lanczos from a random vector")
     ap.add_argument("--guessed-f64", action="store_true",
                     help="with a mixed precision, solve the steps that start from the guess in FP64")
@@ -94,7 +97,8 @@ def main(argv=None):
     d = DMRG(ctx, a.Lx, a.Ny, t1=a.t1, t2=a.t2, J1=a.J1, J2=a.J2, delta=a.delta, kept_min=a.kept_min,
              kept_max=a.kept_max, kept_maxf=a.kept_maxf, sweeps=a.sweeps, lanzero=a.lanzero, trun_err=a.trun_err,
              prec=prec, refine=a.refine if prec != capi.PREC_F64 else 0, seed=a.seed, log=log,
-             wave_guess=not a.no_guess, guessed_f64=a.guessed_f64, shadow_f64=a.shadow_f64)
+             wave_guess=not a.no_guess, guessed_f64=a.guessed_f64, shadow_f64=a.shadow_f64,
+             renorm_prec=capi.PREC_NAMES[a.renorm_prec])
     E = d.run(checkpoint=a.checkpoint, restart=restart, log_dir=a.log_dir)
     wall = time.perf_counter() - t0
     sweep_rows = [h for h in d.history if h[0].startswith("sweep")]
