@@ -28,4 +28,15 @@ parameters):
                       n n = n, sum_s c+_s c_s = n), two-site operators equal
                       the physical ones; the standard Clebsch-Gordan phase in
                       its place flips the pair operator (the check fails)
+  ref_blocks.*        the reference's block states as Fock vectors: every
+                      enlarged state is a multiplet (each operator's
+                      components fit the Wigner-Eckart form to 1e-10), the
+                      block Hamiltonian's reduced matrix is that of the sum
+                      of its bonds, the stored ground state is an exact
+                      eigenvector of the whole-lattice H and stops being one
+                      when the env sign is dropped; inb1 against the closed
+                      rule (bonds leaving the block, plus sites whose last
+                      neighbour is the block's last site). Parity unpinned:
+                      the (sys, env) order of the superblock coupling, which
+                      only the reference's model_wave phases would fix
 """

@@ -15,13 +15,10 @@ H = sum_<ij> [ -t_ij sum_s (c+_is c_js + h.c.) + J_ij (S_i.S_j - n_i n_j / 4) ]
 """
 
 
-def bonds(Lx, Ny, t1, t2, J1, J2, ncols=None, present=None):
-    """List of (i, j, t, J) with i < j (1-based chain indices).
-
-    ncols: number of columns of the cylinder the neighbour table is built on
-    (defaults to Lx); present: optional set of site indices kept (the
-    warm-up superblock keeps the first and last sites of a shorter
-    cylinder, dmrg_sub.f90:334-370)."""
+def neighbour_table(Lx, Ny, ncols=None):
+    """neib(i, typ) of dmrg_setup (dmrg_sub.f90:442-575, triangle = 1) as
+    {(i, typ): j} for bond types 1-12 (types 13-18 are dropped by the
+    reference itself, their couplings being zero, dmrg_sub.f90:634-643)."""
     nxc = Lx if ncols is None else ncols
     neib = {}
 
@@ -60,6 +57,17 @@ def bonds(Lx, Ny, t1, t2, J1, J2, ncols=None, present=None):
             j2 = neib.get((j1, 5), 0) if j1 else 0
             if j1 and j2:
                 add(ii, j2, 11)
+    return neib
+
+
+def bonds(Lx, Ny, t1, t2, J1, J2, ncols=None, present=None):
+    """List of (i, j, t, J) with i < j (1-based chain indices).
+
+    ncols: number of columns of the cylinder the neighbour table is built on
+    (defaults to Lx); present: optional set of site indices kept (the
+    warm-up superblock keeps the first and last sites of a shorter
+    cylinder, dmrg_sub.f90:334-370)."""
+    neib = neighbour_table(Lx, Ny, ncols)
     out = {}
     for (ii, typ), jj in neib.items():
         if typ % 2 == 0:

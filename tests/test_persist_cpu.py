@@ -50,7 +50,12 @@ def test_round_trips(tmp_path):
     persist.write_total_block(w, tb, secs, 5)
     w.close()
     length, back = persist.read_total_block(persist.RecordReader(p), secs)
-    assert length == 5 and set(back.blocks) == set(tb.blocks)
+    # every (bra, ket) pair the rank and particle change allow is written,
+    # in ket order and then bra 2S ascending, a zero block where the
+    # operator has none
+    # (update_trun_ndia's sub-block list)
+    assert length == 5 and list(back.blocks) == [(1, 2), (0, 2), (2, 3)]
+    assert not back.blocks[(1, 2)].any()
     for k in tb.blocks:
         assert np.array_equal(back.blocks[k], tb.blocks[k])
     # basis text: the formats of basis_to_disk, read back to the same structure
@@ -113,7 +118,8 @@ def test_restart_after_an_interrupted_run(monkeypatch, tmp_path):
         stop.run(checkpoint=str(tmp_path / "b"))
     n_done = len(stop.history)
     again = _dmrg(ctx)
-    loaded = persist.model_from_disk(str(tmp_path / "b" / persist.block_name("sys", 5)))
+    loaded = persist.model_from_disk(str(tmp_path / "b" / persist.block_name("sys", 5)), "sys", list(range(1, 6)),
+                                     stop.slot_sites("sys", 5))
     assert loaded.sites == stop.sys_blocks[5].sites
     e = again.run(checkpoint=str(tmp_path / "b"), restart=(2, 0, 0))
     assert again.history[:n_done] == [tuple(h) for h in stop.history]

@@ -168,14 +168,26 @@ class DMRG:
         if self.log:
             self.log(msg)
 
-    def _keep_sites(self, sites_in_block, extra_partners=()):
+    def _keep_sites(self, sites_in_block, extra_partners=(), side=None):
+        """Sites whose operators the grown block carries: those with a bond
+        or a partner outside it, and, when the run writes block files, also
+        every site of the block's operator slots (slot_sites), which the
+        reference's model files hold."""
         inside = set(sites_in_block)
+        slots = set(self.slot_sites(side, len(sites_in_block))) if (side and self.checkpoint) else set()
         keep = []
         for i in sites_in_block:
             partners = [j for (j, _, _) in self.nb.get(i, [])] + [j for (a, j) in extra_partners if a == i]
-            if any(j not in inside for j in partners):
+            if i in slots or any(j not in inside for j in partners):
                 keep.append(i)
         return keep
+
+    def slot_sites(self, side, L):
+        """Chain sites of the operator slots of a `side` ("sys"/"env") block
+        of L sites, in slot order (lattice.op_slots; an env block counts its
+        sites from the chain's far end)."""
+        local = lattice.op_slots(self.bonds, L)
+        return local if side == "sys" else [self.N + 1 - i for i in local]
 
     def _truncate(self, sb, psi_vec, side, kept_min, kept_max, trun_err, identity):
         """U per enlarged sector and the new sector order of the grown block."""
@@ -345,7 +357,7 @@ class DMRG:
             U, order, err, ns = self._truncate(sb, psi, 0, self.kept_min, kept_max, trun_err, identity_sys)
             t0 = time.perf_counter()
             to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(s1, sys.sites, bonds)]
-            keep = self._keep_sites(sys.sites + [s1], partners)
+            keep = self._keep_sites(sys.sites + [s1], partners, "sys")
             out["sys"] = ham.renormalize(self.ctx, sys, s1, to_new, keep, U, order)
             out["sys_trun"] = wavetrans.Trunc(sb.sys_bs, U, order)
             self.timers["renorm"] += time.perf_counter() - t0
@@ -354,7 +366,7 @@ class DMRG:
             U, order, err, ns = self._truncate(sb, psi, 1, self.kept_min, kept_max, trun_err, identity_env)
             t0 = time.perf_counter()
             to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(s2, env.sites, bonds)]
-            keep = self._keep_sites(env.sites + [s2], partners)
+            keep = self._keep_sites(env.sites + [s2], partners, "env")
             out["env"] = ham.renormalize(self.ctx, env, s2, to_new, keep, U, order)
             out["env_trun"] = wavetrans.Trunc(sb.env_bs, U, order)
             self.timers["renorm"] += time.perf_counter() - t0
@@ -380,7 +392,7 @@ class DMRG:
         for side, blk, site in (("sys", sysd, s1), ("env", envd, s2)):
             if grows[side]:
                 to_new = [(i, t, J) for (i, j, t, J) in self._pairs_with(site, blk.sites, bonds)]
-                keep = self._keep_sites(blk.sites + [site], partners)
+                keep = self._keep_sites(blk.sites + [site], partners, side)
                 spec[side] = (blk, site, to_new, keep, ham.renorm_pairs(blk, to_new))
         step = resident.StepArena(ctx, [sysd, envd])
         made = {}   # device objects of this step, recorded as they appear
@@ -694,7 +706,8 @@ class DMRG:
         from . import persist
         blocks, truns = (self.sys_blocks, self.sys_trun) if side == "sys" else (self.env_blocks, self.env_trun)
         d = self.checkpoint
-        persist.model_to_disk(os.path.join(d, persist.block_name(side, length)), blocks[length])
+        persist.model_to_disk(os.path.join(d, persist.block_name(side, length)), blocks[length], side,
+                              self.slot_sites(side, length))
         tr = truns[length]
         persist.truns_to_disk(os.path.join(d, persist.trun_name(side, length)), tr, length, env=(side == "env"))
         persist.basis_to_disk(os.path.join(d, persist.basis_name(side, length)), tr.basis, length)
@@ -734,7 +747,8 @@ class DMRG:
                 f = os.path.join(path, persist.block_name(side, L))
                 if not os.path.exists(f):
                     continue
-                blocks[L] = persist.model_from_disk(f)
+                sites = list(range(1, L + 1)) if side == "sys" else list(range(self.N, self.N - L, -1))
+                blocks[L] = persist.model_from_disk(f, side, sites, self.slot_sites(side, L))
                 prev = blocks.get(L - 1)
                 if prev is not None:
                     # a block that was built from an earlier block of length

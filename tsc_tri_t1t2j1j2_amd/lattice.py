@@ -74,3 +74,19 @@ def neighbour(i, typ, Lx, Ny):
     if typ == 5:
         return (iy - 2) % Ny + 1 + ix * Ny
     raise ValueError(f"bond type {typ} is not one of 1, 3, 5")
+
+
+def op_slots(bond_list, L):
+    """Operator slots of a block of L sites (INB1 / nleg11 of dmrg_setup0,
+    dmrg_sub.f90:304-331): the block's sites, numbered 1..L from the end of
+    the chain it grows from, that have a neighbour numbered L or higher, in
+    increasing order; slot k (1-based) of the block's model file holds the
+    operators of the k-th of them. bond_list: bonds() of the full cylinder
+    (its couplings decide which neighbour shells exist). The same table
+    serves the env block, whose site numbered II from the far end is chain
+    site N + 1 - II (the cylinder is symmetric under that map)."""
+    reach = {}
+    for (a, b, _t, _J) in bond_list:
+        reach[a] = max(reach.get(a, 0), b)
+        reach[b] = max(reach.get(b, 0), a)
+    return [i for i in range(1, L + 1) if reach.get(i, 0) >= L]

@@ -18,18 +18,23 @@ record layouts:
   record, (sys N, 2S), (env N, 2S), (sys dim, env dim), the psi block;
 * systruns<L> / envtruns<L>: one Total_Block, a sub-block per kept enlarged
   sector (its N, 2S; rows = enlarged dim, cols = kept);
-* sysbs<L> / envbs<L>: the formatted Total_Basis text.
+* sysbs<L> / envbs<L>: the formatted Total_Basis text;
+* sysblocks<L> / envblocks<L> (model_to_disk): len, the block Hamiltonian,
+  then per operator slot (INB1 / nleg11 of dmrg_setup0, lattice.op_slots)
+  elec_up, elec_down, spin_sd, num_sn.
 
-Numbers follow this build's conventions: env operators act as psi . O^T, so
-env operators and env truncation matrices are the complex conjugates of the
-reference's; the writers store env truncations conjugated (the reference's
-convention) and the readers undo it. Block operator files (sysblocks<L>)
-hold this build's operators (c+, S, n with the reduced elements of
-blocks.site_operators) in Total_Block records, one triple per site the
-block still carries, listed in a sidecar JSON file; they restart this
-driver, they are not interchangeable with the reference's model files.
+Contents are the reference's numbers, converted from this build's
+conventions where they differ: env operators here act as psi . O^T, so env
+operators and env truncation matrices are the complex conjugates of the
+reference's, and an enlarged env basis here orders its fermion modes block
+then site where the reference puts the site first, which flips the sign of
+the product states with an odd block and an odd site (env_signs): env
+truncation rows and the env columns of a wave function carry that sign.
+Writers apply both conversions and readers undo them. The reference's
+model files also hold elec_down, which this build derives from c+
+(elec_down), and slots for sites whose last neighbour is the block's last
+site; a checkpointed run carries those sites (DMRG.slot_sites).
 """
-import json
 import struct
 
 import numpy as np
@@ -113,15 +118,28 @@ def _dense(m):
 
 def write_total_block(w, tb, sectors, length):
     """block_to_disk (block.f90:1715-1737) of an operator on a block with
-    sectors [(N, 2S, dim)]; sub-blocks in (ket, bra) order."""
-    keys = sorted(tb.blocks, key=lambda k: (k[1], k[0]))
+    sectors [(N, 2S, dim)]. Sub-blocks in the order update_trun_dia /
+    update_trun_ndia leave them (block.f90:3465-3620): for each ket sector
+    in block order, the bra sectors of 2S from 2S_ket - 2k to 2S_ket + 2k
+    that exist, a zero matrix where the operator has no such block."""
+    index = {(s[0], s[1]): i for i, s in enumerate(sectors)}
+    keys = []
+    for k, (n, j2, _d) in enumerate(sectors):
+        for jb in range(j2 - tb.k2, j2 + tb.k2 + 1, 2):
+            b = index.get((n + tb.up_dif, jb))
+            if b is not None:
+                keys.append((b, k))
+    extra = set(tb.blocks) - set(keys)
+    if extra:
+        raise ValueError(f"operator blocks {sorted(extra)} do not match its sectors")
     w.record(int(length))
     w.record(len(keys))
     w.record(int(sum(sectors[k][2] for (_, k) in keys)))
     w.record(int(tb.up_dif))
     w.record(int(tb.k2))
     for (b, k) in keys:
-        m = _dense(tb.blocks[(b, k)])
+        m = tb.blocks.get((b, k))
+        m = np.zeros((sectors[b][2], sectors[k][2]), np.complex128) if m is None else _dense(m)
         w.record()
         w.record(int(sectors[k][0]), int(sectors[k][1]), int(sectors[b][1] - sectors[k][1]))
         w.record(int(m.shape[0]), int(m.shape[1]))
@@ -147,6 +165,29 @@ def read_total_block(r, sectors):
 # ---------------------------------------------------------------------
 # wave function, truncations, bases
 
+def env_signs(basis):
+    """Per enlarged-env sector x, the +-1 vector over its rows that takes
+    this build's env basis to the reference's: (-1)^(N_block N_site).
+
+    This build orders the fermion modes of an enlarged env as block then
+    site, like the sys side; the reference puts the site first (its
+    update_block_ndia_env gives block operators the sign (-1)^N_site and
+    update_site_ndia_env gives site operators none, block.f90:2828-3459,
+    against (-1)^N_block on site operators in update_site_ndia_sys). The
+    two product states differ by exactly that sign; everything else (the
+    6j embedding, the coupling order block (x) site) is the same, so
+    renormalised env operators agree and only the rows of an env
+    truncation and the env columns of a wave function change."""
+    out = []
+    for (_N, _J, d, subs) in basis.sectors:
+        v = np.ones(d)
+        for (a, st, p, sd) in subs:
+            if basis.block_sectors[a][0] % 2 and basis.site_sectors[st][0] % 2:
+                v[p:p + sd] = -1.0
+        out.append(v)
+    return out
+
+
 def wave_to_disk(path, super_basis, sys_bs, env_bs, psi, sys_len, env_len):
     """wave_to_disk (block.f90:1547-1576): psi (host vector in the super
     basis layout) of a superblock whose sys part has sys_len sites (block +
@@ -158,13 +199,14 @@ def wave_to_disk(path, super_basis, sys_bs, env_bs, psi, sys_len, env_len):
         w.record(super_basis.num)
         w.record(int(super_basis.dim))
         psi = np.asarray(psi, np.complex128)
+        sg = env_signs(env_bs)
         for i, (x, y, dx, dy) in enumerate(super_basis.sub):
             w.record()
             w.record(sys_bs.sectors[x][0], sys_bs.sectors[x][1])
             w.record(env_bs.sectors[y][0], env_bs.sectors[y][1])
             w.record(dx, dy)
             o = super_basis.offset[i]
-            w.record(psi[o:o + dx * dy].reshape((dx, dy), order="F"))
+            w.record(psi[o:o + dx * dy].reshape((dx, dy), order="F") * sg[y][None, :])
     finally:
         w.close()
 
@@ -186,6 +228,7 @@ def wave_from_disk(path):
 def psi_in_super_basis(subs, super_basis, sys_bs, env_bs):
     """The host psi vector of super_basis from wave_from_disk's sectors."""
     psi = np.zeros(super_basis.dim, np.complex128)
+    sg = env_signs(env_bs)
     got = {(sq, eq): m for (sq, eq, m) in subs}
     if len(got) != super_basis.num:
         raise ValueError("stored wave function has another super basis")
@@ -194,16 +237,18 @@ def psi_in_super_basis(subs, super_basis, sys_bs, env_bs):
         if m.shape != (dx, dy):
             raise ValueError("stored wave function sector of another shape")
         o = super_basis.offset[i]
-        psi[o:o + dx * dy] = m.ravel(order="F")
+        psi[o:o + dx * dy] = (m * sg[y][None, :]).ravel(order="F")
     return psi
 
 
 def truns_to_disk(path, trun, length, env=False):
     """truns_to_disk (block.f90:1935-1951) of the truncation that built a
-    block of `length` sites; env truncations are stored conjugated."""
+    block of `length` sites; env truncations are stored conjugated and with
+    the rows in the reference's fermion order (env_signs)."""
     w = RecordWriter(path)
     try:
         bs = trun.basis
+        sg = env_signs(bs) if env else None
         w.record(int(length))
         w.record(len(trun.order))
         w.record(int(sum(trun.U[x].shape[1] for x in trun.order)))
@@ -214,7 +259,7 @@ def truns_to_disk(path, trun, length, env=False):
             w.record()
             w.record(bs.sectors[x][0], bs.sectors[x][1], 0)
             w.record(u.shape[0], u.shape[1])
-            w.record(np.conj(u) if env else u)
+            w.record(sg[x][:, None] * np.conj(u) if env else u)
     finally:
         w.close()
 
@@ -223,6 +268,7 @@ def truns_from_disk(path, basis, env=False):
     """A Trunc over `basis` (the enlarged basis the truncation acts on)."""
     r = RecordReader(path)
     length, num, _dim, _ud, _dd = (r.int1() for _ in range(5))
+    sg = env_signs(basis) if env else None
     U, order = {}, []
     for _ in range(num):
         r.empty()
@@ -232,7 +278,7 @@ def truns_from_disk(path, basis, env=False):
         if rows != basis.sectors[x][2]:
             raise ValueError(f"truncation of sector {(n_up, n_down)} has {rows} rows, the basis {basis.sectors[x][2]}")
         u = r.matrix(rows, cols)
-        U[x] = np.conj(u) if env else u
+        U[x] = sg[x][:, None] * np.conj(u) if env else u
         order.append(x)
     return length, Trunc(basis, U, order)
 
@@ -282,42 +328,107 @@ def basis_from_disk(path):
 
 
 # ---------------------------------------------------------------------
-# blocks of this driver
+# block (model) files
 
-def model_to_disk(path, model):
-    """The block Hamiltonian and the (c+, S, n) operators of every site the
-    block carries, as Total_Block records (model_to_disk's layout,
-    block.f90:1802-1844); the sites and sectors in path + '.json'."""
+def _par(x2):
+    """(-1)^(x2/2) for an even doubled exponent."""
+    return -1.0 if (x2 // 2) % 2 else 1.0
+
+
+def elec_down(c, sectors):
+    """The reference's elec_down (the rank-1/2 annihilation tensor c~,
+    with c~_{+1/2} = -c_down and c~_{-1/2} = c_up, st_elec_down of
+    dmrg_sub.f90:779-791) from its elec_up c+ on a block with sectors
+    [(N, 2S, dim)]. With the Wigner-Eckart convention <J'M'|T^k_q|JM> =
+    (-1)^(J'-M') (J' k J; -M' q M) <J'||T||J> and c~_q = (-1)^(k+q)
+    (c+_{-q})^dagger, a symmetry of the 3j symbol gives
+    <J'||c~||J> = (-1)^(3/2 + J - J') conj(<J||c+||J'>)."""
+    out = TotalBlock(-c.up_dif, c.k2)
+    for (b, k), m in c.blocks.items():
+        # c+ block <b||c+||k>: c~ block with bra k, ket b
+        out.blocks[(k, b)] = _par(sectors[b][1] - sectors[k][1] + 3) * np.conj(_dense(m)).T
+    return out
+
+
+def _conj_block(tb):
+    return TotalBlock(tb.up_dif, tb.k2, {k: np.conj(_dense(v)) for k, v in tb.blocks.items()})
+
+
+def model_to_disk(path, model, side, slots):
+    """model_to_disk (block.f90:1802-1844) in the reference's layout and
+    conventions: len; the block Hamiltonian; then for each operator slot
+    (slots: chain sites in slot order, lattice.op_slots) the slot site's
+    elec_up (c+), elec_down (c~), spin_sd (S) and num_sn (n), the four
+    Total_Blocks model_to_disk writes for the t-J model (tjmodel = 1,
+    v123 = 1, no ring terms). Env blocks are written conjugated (this
+    build's env operators act as psi . O^T). Sites the block carries
+    beyond the slots (partners of measured pairs) follow as a record
+    holding their count and chain numbers and their (c+, S, n) blocks; the
+    reference's reader stops after the slots and never sees them."""
     m = model_to_host(model)
+    env = side == "env"
+    missing = [i for i in slots if i not in m.ops]
+    if missing:
+        raise ValueError(f"{side} block of {m.len} sites carries no operators of slot sites {missing}")
+    conv = _conj_block if env else (lambda tb: tb)
+    L = len(m.sites)
     w = RecordWriter(path)
     try:
-        w.record(len(m.sites))
-        write_total_block(w, m.ham, m.sectors, len(m.sites))
-        op_sites = sorted(m.ops)
-        for i in op_sites:
-            for name in NAMES:
-                write_total_block(w, m.ops[i][name], m.sectors, len(m.sites))
+        w.record(L)
+        write_total_block(w, conv(m.ham), m.sectors, L)
+        for i in slots:
+            up = conv(m.ops[i]["c"])
+            for tb in (up, elec_down(up, m.sectors), conv(m.ops[i]["S"]), conv(m.ops[i]["n"])):
+                write_total_block(w, tb, m.sectors, L)
+        extra = sorted(set(m.ops) - set(slots))
+        if extra:
+            w.record(len(extra), *extra)
+            for i in extra:
+                for name in NAMES:
+                    write_total_block(w, conv(m.ops[i][name]), m.sectors, L)
     finally:
         w.close()
-    with open(path + ".json", "w") as fh:
-        json.dump({"sites": m.sites, "sectors": [list(s) for s in m.sectors], "op_sites": op_sites}, fh)
 
 
-def model_from_disk(path):
-    with open(path + ".json") as fh:
-        meta = json.load(fh)
-    sectors = [tuple(s) for s in meta["sectors"]]
+def model_from_disk(path, side, sites, slots):
+    """model_from_disk (block.f90:1848-1898) of a file in the reference's
+    layout, as a TotalModel: sectors from the Hamiltonian's sub-blocks,
+    the slot sites' c+, S, n (elec_down is implied by c+), and the extra
+    sites model_to_disk appends. sites: the block's chain sites in this
+    build's fermion order (sys 1..L, env N, N-1, ..); slots: its slot sites
+    (lattice.op_slots)."""
+    env = side == "env"
+    conv = _conj_block if env else (lambda tb: tb)
     r = RecordReader(path)
-    length = r.int1()
-    if length != len(meta["sites"]):
-        raise ValueError("block file and its site list disagree")
-    _, ham = read_total_block(r, sectors)
+    L = r.int1()
+    if L != len(sites):
+        raise ValueError(f"block file of {L} sites, expected {len(sites)}")
+    # the Hamiltonian's header and sub-blocks give the sectors
+    _len, num, _dim, up_dif, k2 = (r.int1() for _ in range(5))
+    if (up_dif, k2) != (0, 0):
+        raise ValueError("the first block of a model file is not a scalar")
+    sectors, hblocks = [], {}
+    for x in range(num):
+        r.empty()
+        n, j2, dd = r.ints(3)
+        rows, cols = r.ints(2)
+        if dd != 0 or rows != cols:
+            raise ValueError("Hamiltonian sub-block off the diagonal")
+        sectors.append((n, j2, rows))
+        hblocks[(x, x)] = r.matrix(rows, cols)
+    ham = conv(TotalBlock(0, 0, hblocks))
     ops = {}
-    for i in meta["op_sites"]:
-        ops[i] = {}
-        for name in NAMES:
-            _, ops[i][name] = read_total_block(r, sectors)
-    return TotalModel(meta["sites"], sectors, ham, ops)
+    for i in slots:
+        _, up = read_total_block(r, sectors)
+        read_total_block(r, sectors)          # elec_down
+        _, S = read_total_block(r, sectors)
+        _, n = read_total_block(r, sectors)
+        ops[i] = {"c": conv(up), "S": conv(S), "n": conv(n)}
+    if r.pos < len(r.data):
+        extra = r.ints()
+        for i in extra[1:1 + extra[0]]:
+            ops[i] = {name: conv(read_total_block(r, sectors)[1]) for name in NAMES}
+    return TotalModel(list(sites), sectors, ham, ops)
 
 
 # ---------------------------------------------------------------------
