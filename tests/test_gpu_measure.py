@@ -90,3 +90,30 @@ def test_device_checkpoint_and_measurement_replay(ctx, tmp_path):
     again = DMRG(ctx, 4, 3, Ne=10, kept_min=20, kept_max=40, sweeps=1, lanzero=1e-10, trun_err=1e-12)
     e2 = again.run(checkpoint=str(tmp_path), restart=(2, 0, 0))
     assert abs(e2 - e) < 1e-6, (e2, e)
+
+
+def test_device_block_files_are_the_references(ctx, tmp_path):
+    """The resident device path (enlargement and U^H O U on the GPU) writes
+    block, truncation and wave-function files whose every matrix equals the
+    reduced matrix of the physical operator between the reference's block
+    states (oracle/ref_blocks.py): a 4 x 3 warm-up with truncation, both
+    sides, and the ground state of a 2 x 3 warm-up."""
+    import ref_file_checks as C
+    pytest.importorskip("sympy")
+    cpl = (1.0, 0.5, 0.4, 0.2)
+    d = DMRG(ctx, 4, 3, t1=cpl[0], t2=cpl[1], J1=cpl[2], J2=cpl[3], Ne=10, kept_min=4, kept_max=7, sweeps=1,
+             lanzero=1e-10, trun_err=1e-14)
+    d.checkpoint = str(tmp_path / "a")
+    import os
+    os.makedirs(d.checkpoint)
+    d.warm_up()
+    assert C.check_block_files(d.checkpoint, "sys", 4, 3, cpl, d.N // 2) == 0
+    assert C.check_block_files(d.checkpoint, "env", 4, 3, cpl, d.N // 2) > 0
+    w = DMRG(ctx, 2, 3, t1=cpl[0], t2=cpl[1], J1=cpl[2], J2=cpl[3], Ne=4, kept_min=4, kept_max=40, sweeps=1,
+             lanzero=1e-12, trun_err=1e-14)
+    w.checkpoint = str(tmp_path / "b")
+    os.makedirs(w.checkpoint)
+    w.warm_up()
+    nrm, Eh, res, res_bad = C.wave_check(w.checkpoint, 6, 2, 3, cpl, w.history[-1][2])
+    assert abs(nrm - 1.0) < 1e-10 and abs(Eh - w.history[-1][2]) < 1e-9
+    assert res < 1e-6 and res_bad > 1e-3
