@@ -36,7 +36,9 @@ parameters):
                       eigenvector of the whole-lattice H and stops being one
                       when the env sign is dropped; inb1 against the closed
                       rule (bonds leaving the block, plus sites whose last
-                      neighbour is the block's last site). Parity unpinned:
-                      the (sys, env) order of the superblock coupling, which
-                      only the reference's model_wave phases would fix
+                      neighbour is the block's last site); the restated
+                      model_wave site-site hopping and Get_hamiltonian_ndia
+                      block-site hopping are both 1/sqrt2 times the physical
+                      hopping in these states (the opposite for the cross
+                      term under the other superblock coupling order)
 """

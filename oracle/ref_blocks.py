@@ -44,6 +44,7 @@ import numpy as np
 from . import ref_measure as R
 from .su2_dense import get_basis
 from .tj_model import neighbour_table
+from .wigner import w6j
 
 SITE_SECTORS = [(1, 1, 1), (0, 0, 1)]      # electron doublet, hole
 
@@ -207,4 +208,111 @@ def super_state(sys_states, env_states, subs):
     return psi
 
 
-__all__ = ["inb1", "BlockStates", "reduced", "fock_ops", "block_hamiltonian", "super_state"]
+def _pm(x2):
+    """(-1)^(x2/2) for an even doubled exponent."""
+    assert x2 % 2 == 0
+    return -1.0 if (x2 // 2) % 2 else 1.0
+
+
+# the site operators both hopping routines receive for the t-J model:
+# st_elec_up for the block / sys site and block_transfer_trans(st_elec_down)
+# for the other side (get_sys.f90:56-57, 6963-6965); as data both are one
+# sub-block, ket hole (0, 0), down_dif +1, value -sqrt2 (dmrg_sub.f90:759-791,
+# block.f90:33-64 transposes and conjugates st_elec_down's single element)
+_SITE_HOP = -np.sqrt(2.0)
+
+
+def ref_block_site_hop(eb, cplus, coef):
+    """Get_hamiltonian_ndia(block = elec_up of a block site, site =
+    trans(st_elec_down), basis, coef, ham, 'F', 'S') (get_sys.f90:865-1056):
+    the hopping between a sys block site and the new site added to the
+    enlarged block's Hamiltonian, as {(x, x): reduced matrix} over the
+    enlarged basis eb (BlockStates.enlarge, sys side). cplus: {(bra, ket):
+    reduced matrix} of the block site's c+ between the block's sectors
+    (eb.block_sectors)."""
+    bsec = eb.block_sectors
+    H = {}
+    for x, (N, J, d, subs) in enumerate(eb.basis):
+        M = np.zeros((d, d), complex)
+        for (a, st, p, sd) in subs:                     # ket: block a, site st
+            nb, jb, _ = bsec[a]
+            n_st, j_st, _ = SITE_SECTORS[st]
+            for (ab, ak), blk in cplus.items():
+                if ak != a:
+                    continue
+                nb2, jb2, _ = bsec[ab]
+                # the site loses its electron: ket site electron, bra hole
+                if n_st != 1:
+                    continue
+                j_new = 0
+                for (a2, st2, p2, sd2) in subs:        # bra sub in the same sector
+                    if a2 != ab or SITE_SECTORS[st2][1] != j_new or SITE_SECTORS[st2][0] != 0:
+                        continue
+                    k = 1
+                    c11 = w6j(jb2, jb, k, j_st, j_new, J)
+                    c11 *= np.sqrt((1.0 + J) / (1.0 + k)) * _pm(jb + j_new + k + J) * _SITE_HOP
+                    sign = -1.0 if nb % 2 == 1 else 1.0     # (-1)^N_block of the ket, 'S'
+                    c1 = c11 * coef * sign
+                    if c1 == 0.0:
+                        continue
+                    M[p2:p2 + sd2, p:p + sd] += c1 * blk
+                    M[p:p + sd, p2:p2 + sd2] += np.conj(c1) * np.conj(blk).T
+        H[(x, x)] = M
+    return H
+
+
+def ref_site_site_hop(sys_eb, env_eb, psi, coef):
+    """site_vec_site_ndia(st_elec_up, 'F', trans(st_elec_down), 'F',
+    sys_bs, env_bs, invec, outvec, coef) (get_sys.f90:4816-5211, called
+    from model_wave, get_sys.f90:6963-6966): the hopping between the sys
+    site and the env site applied to a total-spin-0 wave function psi =
+    {(sys sector, env sector): dx x dy matrix} over the enlarged bases."""
+    out = {key: np.zeros_like(m) for key, m in psi.items()}
+    k = 1
+    for (x, y), vin in psi.items():
+        Ns, Js, _dx, subs_s = sys_eb.basis[x]
+        Ne, Je, _dy, subs_e = env_eb.basis[y]
+        assert Js == Je
+        for (a, st, p, sd) in subs_s:
+            n_bl, j_bl, _ = sys_eb.block_sectors[a]
+            n_st, j_st, _ = SITE_SECTORS[st]
+            if n_st != 0:                      # st_elec_up acts on an empty site
+                continue
+            nsn, jsn = 1, 1                    # new sys site: the electron
+            for (ae, ste, pe, sde) in subs_e:
+                n_ebl, j_ebl, _ = env_eb.block_sectors[ae]
+                n_est, j_est, _ = SITE_SECTORS[ste]
+                if n_est != 1:                 # the env site gives up its electron
+                    continue
+                nen, jen = 0, 0
+                for Jn in range(abs(Js - k), Js + k + 1, 2):
+                    xs = [i for i, b in enumerate(sys_eb.basis) if (b[0], b[1]) == (Ns + 1, Jn)]
+                    ye = [i for i, b in enumerate(env_eb.basis) if (b[0], b[1]) == (Ne - 1, Jn)]
+                    if not xs or not ye or (xs[0], ye[0]) not in psi:
+                        continue
+                    xn, yn = xs[0], ye[0]
+                    ps = [q for (a2, s2, q, _d) in sys_eb.basis[xn][3]
+                          if a2 == a and SITE_SECTORS[s2][:2] == (nsn, jsn)]
+                    pe2 = [q for (a2, s2, q, _d) in env_eb.basis[yn][3]
+                           if a2 == ae and SITE_SECTORS[s2][:2] == (nen, jen)]
+                    if not ps or not pe2:
+                        continue
+                    sign_st = -1.0 if n_bl % 2 else 1.0        # sys site passes the sys block
+                    sign_et = -1.0 if Ns % 2 else 1.0          # env site passes the whole sys part
+                    j1, j2, j3, j4, j5, j6 = Jn, k, Js, j_st, j_bl, jsn
+                    j11, j22, j33, j44, j55, j66 = Jn, k, Je, j_est, j_ebl, jen
+                    c = w6j(j1, j2, j3, j4, j5, j6) * w6j(j11, j22, j33, j44, j55, j66)
+                    if c == 0.0:
+                        continue
+                    c *= np.sqrt((1.0 + j11) * (1.0 + j33) / (1.0 + k))
+                    if ((j55 + j44 + j11 + j22) // 2 + (j5 + j4 + j1 + j2) // 2) % 2:
+                        c = -c
+                    c *= sign_st * sign_et * coef * _SITE_HOP * _SITE_HOP
+                    q_s, q_e = ps[0], pe2[0]
+                    out[(xn, yn)][q_s:q_s + sd, q_e:q_e + sde] += c * vin[p:p + sd, pe:pe + sde]
+                    out[(x, y)][p:p + sd, pe:pe + sde] += np.conj(c) * psi[(xn, yn)][q_s:q_s + sd, q_e:q_e + sde]
+    return out
+
+
+__all__ = ["inb1", "BlockStates", "reduced", "fock_ops", "block_hamiltonian", "super_state", "ref_block_site_hop",
+           "ref_site_site_hop"]

@@ -126,3 +126,62 @@ def test_wave_file_is_the_ground_state(tmp_path):
     assert Eh == pytest.approx(E, abs=1e-9)
     assert res < 1e-6
     assert res_bad > 1e-3
+
+
+def test_superblock_coupling_order_is_the_references():
+    """The SU(2) coupling of the superblock (sys, env) that eigvec1 is read
+    with is the reference's: its model_wave hopping between the sys site and
+    the env site (site_vec_site_ndia, restated in oracle/ref_blocks.py)
+    acts on wave functions read that way as the same multiple of the
+    physical hopping sum_s (c+_3s c_4s + h.c.) as its block-internal hopping
+    (Get_hamiltonian_ndia, restated too) does inside an enlarged block,
+    1/sqrt2 (the coefficient jt = -sqrt2 t of main.f90:64-65 then gives
+    -t). Coupled in the order (env, sys) the wave functions of half-integer
+    spin sectors change sign and the cross hopping comes out with the
+    opposite multiple."""
+    site = RB.BlockStates.site()
+
+    def whole(eb):
+        return eb.truncate([(x, np.eye(d)) for x, (_, _, d) in enumerate(eb.sectors)])
+
+    sys2, env2 = whole(site.enlarge(False)), whole(site.enlarge(True))   # sites 1, 2 and 5, 6
+    sys_eb, env_eb = sys2.enlarge(False), env2.enlarge(True)             # + site 3, + site 4
+    # block-internal: site 2 of the block with the new site 3
+    _F2, ops2 = RB.fock_ops(2)
+    cplus, dev = RB.reduced(sys2, sys2, 1, ops2[1]["c+"][1], 1)
+    assert dev < 1e-12
+    Mb = RB.ref_block_site_hop(sys_eb, cplus, 1.0)
+    X3 = RB.R.physical(RB.R.Fock(3), "hop", 1, 2)
+    Xr, dev = RB.reduced(sys_eb, sys_eb, 0, {0: X3 + X3.T}, 0)
+    assert dev < 1e-12
+    lam_b = (sum(np.vdot(Xr[k], Mb[k]) for k in Mb) / sum(np.vdot(Xr[k], Xr[k]) for k in Mb)).real
+    assert max(np.abs(Mb[k] - lam_b * Xr[k]).max() for k in Mb) < 1e-12
+    assert lam_b == pytest.approx(1.0 / np.sqrt(2.0), rel=1e-12)
+    # across the superblock, on every basis wave function of N = 4, S = 0
+    F6 = RB.R.Fock(6)
+    X6 = RB.R.physical(F6, "hop", 2, 3)
+    X6 = X6 + X6.T
+    keys = [(x, y) for x, bx in enumerate(sys_eb.basis) for y, by in enumerate(env_eb.basis)
+            if bx[0] + by[0] == 4 and bx[1] == by[1]]
+    idx = [(x, y, r, c) for (x, y) in keys for r in range(sys_eb.basis[x][2]) for c in range(env_eb.basis[y][2])]
+    pos = {k: i for i, k in enumerate(idx)}
+    psis, M = [], np.zeros((len(idx), len(idx)), complex)
+    for b, (x, y, r, c) in enumerate(idx):
+        one = {k: np.zeros((sys_eb.basis[k[0]][2], env_eb.basis[k[1]][2]), complex) for k in keys}
+        one[(x, y)][r, c] = 1.0
+        psis.append(RB.super_state(sys_eb, env_eb, [(x, y, one[(x, y)])]))
+        for k, m in RB.ref_site_site_hop(sys_eb, env_eb, one, 1.0).items():
+            for (rr, cc) in zip(*np.nonzero(m)):
+                M[pos[(k[0], k[1], rr, cc)], b] = m[rr, cc]
+    P = np.array(psis)
+    assert np.abs(P.conj() @ P.T - np.eye(len(idx))).max() < 1e-12
+    for order, flip in (("sys, env", False), ("env, sys", True)):
+        s = np.array([(-1.0) ** sys_eb.basis[x][1] if flip else 1.0 for (x, _y, _r, _c) in idx])
+        Q = P * s[:, None]
+        X = Q.conj() @ X6 @ Q.T
+        lam = (np.vdot(X, M) / np.vdot(X, X)).real
+        assert np.abs(M - lam * X).max() < 1e-12, order
+        if flip:
+            assert lam == pytest.approx(-lam_b, rel=1e-12)
+        else:
+            assert lam == pytest.approx(lam_b, rel=1e-12)
