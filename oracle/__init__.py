@@ -41,4 +41,9 @@ parameters):
                       block-site hopping are both 1/sqrt2 times the physical
                       hopping in these states (the opposite for the cross
                       term under the other superblock coupling order)
+  ref_kernels.*       the 14 model_wave kernels restated from the Fortran:
+                      each is one constant times its physical term on every
+                      basis wave function of a 6-site superblock (residual
+                      1e-12), the constant the couplings jt, jd, jn assume;
+                      a dropped block_transfer_trans or env sign breaks it
 """

@@ -1,0 +1,189 @@
+"""The reference's 14 superblock kernels act as physical operators on the
+wave functions this build stores.
+
+Each kernel of model_wave (get_sys.f90:1783-5204), restated loop by loop in
+oracle/ref_kernels.py with the reference's own 6j coefficients, phases and
+fermion signs, is applied to every basis wave function of a 6-site
+superblock (sys block 1-2, sys site 3, env site 4, env block 5-6; N = 4,
+S = 0) built in the reference's conventions (oracle/ref_blocks.py), with
+block operators given as the reference stores them. Each must equal one
+constant times the physical operator of its term, with the constant that
+model_wave's couplings assume (main.f90:61-73): jt = -sqrt2 t needs
+1/sqrt2 for sum_s (c+_is c_js + h.c.), jd = -sqrt3 J needs -1/sqrt3 for
+S_i.S_j, jn = -J/4 needs 1 for n_i n_j, and the block Hamiltonians enter
+with 1. This pins the reference's per-kernel phases and the superblock
+conventions of eigvec1 together: a sign slip in any kernel, or a wave
+function coupled or ordered differently, leaves a residual or changes a
+constant. The product's H.psi is pinned to the same physical Hamiltonian
+by exact diagonalisation (test_gpu_physics.py, test_physics_plan.py), so on
+the same blocks it applies what model_wave applies.
+"""
+import numpy as np
+import pytest
+
+pytest.importorskip("sympy")
+from oracle import ref_blocks as RB  # noqa: E402
+from oracle import ref_kernels as K  # noqa: E402
+
+LAM_HOP, LAM_SPIN, LAM_N = 1.0 / np.sqrt(2.0), -1.0 / np.sqrt(3.0), 1.0
+NAMES = {"elec_up": ("c+", 1, 1), "elec_down": ("c~", -1, 1), "spin_sd": ("S", 0, 2), "num_sn": ("n", 0, 0)}
+
+
+@pytest.fixture(scope="module")
+def sb():
+    site = RB.BlockStates.site()
+
+    def whole(eb):
+        return eb.truncate([(x, np.eye(d)) for x, (_, _, d) in enumerate(eb.sectors)])
+
+    sys2, env2 = whole(site.enlarge(False)), whole(site.enlarge(True))
+    sys_eb, env_eb = sys2.enlarge(False), env2.enlarge(True)
+    _F2, o2 = RB.fock_ops(2)
+
+    def ops(states, loc, env):
+        out = {}
+        for name, (fock, up, k) in NAMES.items():
+            red, dev = RB.reduced(states, states, k, o2[loc][fock][1], up)
+            assert dev < 1e-12
+            out[name] = K.from_reduced(red, states.sectors, up, k, env)
+        return out
+
+    t, J = 0.7, 0.3
+    hams = []
+    for states, env in ((sys2, False), (env2, True)):
+        red, dev = RB.reduced(states, states, 0, RB.block_hamiltonian(RB.fock_ops(2)[0], [(0, 1, t, J)]), 0)
+        hams.append(K.from_reduced(red, states.sectors, 0, 0, env))
+    keys = [(x, y) for x, bx in enumerate(sys_eb.basis) for y, by in enumerate(env_eb.basis)
+            if bx[0] + by[0] == 4 and bx[1] == by[1]]
+    idx = [(x, y, r, c) for (x, y) in keys for r in range(sys_eb.basis[x][2]) for c in range(env_eb.basis[y][2])]
+    P = []
+    for (x, y, r, c) in idx:
+        m = np.zeros((sys_eb.basis[x][2], env_eb.basis[y][2]))
+        m[r, c] = 1.0
+        P.append(RB.super_state(sys_eb, env_eb, [(x, y, m)]))
+    P = np.array(P)
+    assert np.abs(P.conj() @ P.T - np.eye(len(idx))).max() < 1e-12
+    return {"sys_eb": sys_eb, "env_eb": env_eb, "keys": keys, "idx": idx, "P": P, "F": RB.R.Fock(6),
+            "sys": {1: ops(sys2, 0, False), 2: ops(sys2, 1, False)},
+            "env": {5: ops(env2, 0, True), 6: ops(env2, 1, True)}, "hams": hams, "tJ": (t, J)}
+
+
+def _matrix(sb, fn):
+    pos = {k: i for i, k in enumerate(sb["idx"])}
+    M = np.zeros((len(sb["idx"]),) * 2, complex)
+    for b, (x, y, r, c) in enumerate(sb["idx"]):
+        one = {k: np.zeros((sb["sys_eb"].basis[k[0]][2], sb["env_eb"].basis[k[1]][2]), complex) for k in sb["keys"]}
+        one[(x, y)][r, c] = 1.0
+        for k, m in fn(one).items():
+            for (rr, cc) in zip(*np.nonzero(m)):
+                M[pos[(k[0], k[1], rr, cc)], b] = m[rr, cc]
+    return M
+
+
+def _phys(sb, kind, a, b):
+    X = RB.R.physical(sb["F"], kind, a - 1, b - 1)
+    return X + X.T if kind == "hop" else X
+
+
+def _constant(sb, fn, X):
+    """c with kernel = c X on the wave functions, asserting the residual."""
+    M = _matrix(sb, fn)
+    XM = sb["P"].conj() @ X @ sb["P"].T
+    assert np.abs(XM).max() > 0.1
+    lam = (np.vdot(XM, M) / np.vdot(XM, XM)).real
+    assert np.abs(M - lam * XM).max() < 1e-12
+    return lam
+
+
+def _cases(sb):
+    """(name, kernel applied to psi, physical operator, constant)."""
+    se, ee = sb["sys_eb"], sb["env_eb"]
+    up, dn, sd, num = (K.site_op(n) for n in ("st_elec_up", "st_elec_down", "st_sd", "num_elec"))
+    T = K.trans
+    S, E = sb["sys"], sb["env"]
+    out = [("site_vec_site_ndia hop 3-4", lambda p: K.site_vec_site_ndia(up, "F", T(dn), "F", se, ee, p, 1.0),
+            ("hop", 3, 4), LAM_HOP),
+           ("site_vec_site_ndia S.S 3-4", lambda p: K.site_vec_site_ndia(sd, "B", sd, "B", se, ee, p, 1.0),
+            ("SS", 3, 4), LAM_SPIN),
+           ("site_vec_site_dia n n 3-4", lambda p: K.site_vec_site_dia(num, num, se, ee, p, 1.0), ("nn", 3, 4), LAM_N)]
+    for i in (1, 2):
+        o = S[i]
+        out += [(f"block_site_vec_ndia hop {i}-3",
+                 lambda p, o=o: K.block_site_vec_ndia(o["elec_up"], "F", T(dn), "F", se, p, 1.0), ("hop", i, 3), LAM_HOP),
+                (f"block_site_vec_ndia S.S {i}-3",
+                 lambda p, o=o: K.block_site_vec_ndia(o["spin_sd"], "B", sd, "B", se, p, 1.0), ("SS", i, 3), LAM_SPIN),
+                (f"block_site_vec_dia n n {i}-3",
+                 lambda p, o=o: K.block_site_vec_dia(o["num_sn"], num, se, p, 1.0), ("nn", i, 3), LAM_N),
+                (f"block_vec_site_ndia hop {i}-4",
+                 lambda p, o=o: K.block_vec_site_ndia(o["elec_up"], "F", T(dn), "F", se, ee, p, 1.0), ("hop", i, 4),
+                 LAM_HOP),
+                (f"block_vec_site_ndia S.S {i}-4",
+                 lambda p, o=o: K.block_vec_site_ndia(o["spin_sd"], "B", sd, "B", se, ee, p, 1.0), ("SS", i, 4),
+                 LAM_SPIN),
+                (f"block_vec_site_dia n n {i}-4",
+                 lambda p, o=o: K.block_vec_site_dia(o["num_sn"], num, se, ee, p, 1.0), ("nn", i, 4), LAM_N)]
+        for j in (5, 6):
+            q = E[j]
+            out += [(f"block_vec_block_ndia hop {i}-{j}",
+                     lambda p, o=o, q=q: K.block_vec_block_ndia(o["elec_up"], "F", T(q["elec_down"]), "F", se, ee, p, 1.0),
+                     ("hop", i, j), LAM_HOP),
+                    (f"block_vec_block_ndia S.S {i}-{j}",
+                     lambda p, o=o, q=q: K.block_vec_block_ndia(o["spin_sd"], "B", T(q["spin_sd"]), "B", se, ee, p, 1.0),
+                     ("SS", i, j), LAM_SPIN),
+                    (f"block_vec_block_dia n n {i}-{j}",
+                     lambda p, o=o, q=q: K.block_vec_block_dia(o["num_sn"], q["num_sn"], se, ee, p, 1.0), ("nn", i, j),
+                     LAM_N)]
+    for j in (5, 6):
+        q = E[j]
+        out += [(f"vec_block_site_ndia hop {j}-4",
+                 lambda p, q=q: K.vec_block_site_ndia(q["elec_up"], "F", T(dn), "F", ee, se, p, 1.0), ("hop", j, 4),
+                 LAM_HOP),
+                (f"vec_block_site_ndia S.S {j}-4",
+                 lambda p, q=q: K.vec_block_site_ndia(q["spin_sd"], "B", sd, "B", ee, se, p, 1.0), ("SS", j, 4),
+                 LAM_SPIN),
+                (f"vec_block_site_dia n n {j}-4",
+                 lambda p, q=q: K.vec_block_site_dia(q["num_sn"], num, ee, p, 1.0), ("nn", j, 4), LAM_N),
+                (f"site_vec_block_ndia hop 3-{j}",
+                 lambda p, q=q: K.site_vec_block_ndia(up, "F", T(q["elec_down"]), "F", se, ee, p, 1.0), ("hop", 3, j),
+                 LAM_HOP),
+                (f"site_vec_block_ndia S.S 3-{j}",
+                 lambda p, q=q: K.site_vec_block_ndia(sd, "B", T(q["spin_sd"]), "B", se, ee, p, 1.0), ("SS", 3, j),
+                 LAM_SPIN),
+                (f"site_vec_block_dia n n 3-{j}",
+                 lambda p, q=q: K.site_vec_block_dia(num, q["num_sn"], se, ee, p, 1.0), ("nn", 3, j), LAM_N)]
+    return out
+
+
+def test_every_kernel_is_its_physical_term(sb):
+    seen = set()
+    for name, fn, (kind, a, b), want in _cases(sb):
+        lam = _constant(sb, fn, _phys(sb, kind, a, b))
+        assert lam == pytest.approx(want, rel=1e-12), name
+        seen.add(name.split()[0])
+    assert len(seen) == 12
+
+
+def test_block_hamiltonian_kernels(sb):
+    t, J = sb["tJ"]
+    F = sb["F"]
+    Hs = RB.block_hamiltonian(F, [(0, 1, t, J)])[0]
+    He = RB.block_hamiltonian(F, [(4, 5, t, J)])[0]
+    se, ee = sb["sys_eb"], sb["env_eb"]
+    assert _constant(sb, lambda p: K.block_vec_dia(sb["hams"][0], se, p, 1.0), Hs) == pytest.approx(1.0, rel=1e-12)
+    assert _constant(sb, lambda p: K.vec_block_dia(sb["hams"][1], ee, p, 1.0), He) == pytest.approx(1.0, rel=1e-12)
+
+
+def test_a_sign_slip_is_seen(sb):
+    """The check is sensitive: the env block operator passed without
+    block_transfer_trans, or the env block sign (-1)^(N_sys + N_site)
+    dropped, no longer gives a multiple of the physical hopping."""
+    se, ee = sb["sys_eb"], sb["env_eb"]
+    o, q = sb["sys"][2], sb["env"][5]
+    X = _phys(sb, "hop", 2, 5)
+    XM = sb["P"].conj() @ X @ sb["P"].T
+    bad = [lambda p: K.block_vec_block_ndia(o["elec_up"], "F", q["elec_down"], "F", se, ee, p, 1.0),
+           lambda p: K.block_vec_block_ndia(o["elec_up"], "F", K.trans(q["elec_down"]), "B", se, ee, p, 1.0)]
+    for fn in bad:
+        M = _matrix(sb, fn)
+        lam = (np.vdot(XM, M) / np.vdot(XM, XM)).real
+        assert np.abs(M - lam * XM).max() > 1e-3
