@@ -45,5 +45,8 @@ parameters):
                       each is one constant times its physical term on every
                       basis wave function of a 6-site superblock (residual
                       1e-12), the constant the couplings jt, jd, jn assume;
-                      a dropped block_transfer_trans or env sign breaks it
+                      a dropped block_transfer_trans or env sign breaks it;
+                      wave_trans_LR/RL keep a complex random state exactly
+                      under complete complex unitary truncations, and fail
+                      when env block states are read as unconjugated columns
 """

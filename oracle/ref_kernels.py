@@ -456,3 +456,77 @@ def block_vec_block_dia(sys_bl, env_bl, sys_eb, env_eb, psi, coef):
         B = _scalar_on_part(env_eb, y, env_bl, "bl")
         out[(x, y)] += coef * (A @ v @ np.conj(B).T)
     return out
+
+
+# ---------------------------------------------------------------------
+# the wave-function guess between steps (dmrg_sub.f90:1471-1789)
+
+def wave_trans_LR(sys_eb, env_eb, sys_trun, env_trun, old_eb, old_psi, new_keys):
+    """wave_trans_LR (dmrg_sub.f90:1471-1633): psi of the superblock
+    (old_eb = (old enlarged sys, old enlarged env)) carried to the next one
+    of a left-to-right pass: the old enlarged sys truncated by sys_trun
+    ({(N, 2S): U}) becomes the new sys block, the old env site becomes the
+    new sys site, the old env block is expanded back to its enlarged basis
+    by env_trun (the truncation that built it), applied conjugate-transposed.
+    sys_eb / env_eb: the new enlarged bases; new_keys: the new psi sectors."""
+    o_sys, o_env = old_eb
+    new = {key: np.zeros((sys_eb.basis[key[0]][2], env_eb.basis[key[1]][2]), complex) for key in new_keys}
+    for (x, y), v in old_psi.items():
+        ns, js = _qn(o_sys, x)
+        ne, je = _qn(o_env, y)
+        us = sys_trun.get((ns, js))
+        if us is None:
+            continue
+        for (ebl, est, p, sd) in _subs(o_env, y):
+            ue = env_trun.get(ebl)
+            if ue is None:
+                continue
+            for jn in range(abs(js - est[1]), js + est[1] + 1, 2):
+                xs = _sector(sys_eb, ns + est[0], jn)
+                ye = _sector(env_eb, ebl[0], ebl[1])
+                f = _find(sys_eb, xs, (ns, js), est)
+                if f is None or ye is None or (xs, ye) not in new:
+                    continue
+                q, nd = f
+                j12, j3, j4, j123, j34 = js, est[1], ebl[1], jn, je
+                c = _pm(j12 + j3 + j123) / np.sqrt((1.0 + j12) * (1.0 + j123))
+                c *= np.sqrt((1.0 + j123) * (1.0 + j34))
+                c *= _pm(j12 + j3 + j4) * _pm(j3 + j4 - j34)
+                sub = c * (np.conj(us).T @ v[:, p:p + sd])
+                new[(xs, ye)][q:q + nd, :] += sub @ np.conj(ue).T
+    nrm = np.sqrt(sum(np.vdot(m, m).real for m in new.values()))
+    return {k: m / nrm for k, m in new.items()}
+
+
+def wave_trans_RL(sys_eb, env_eb, sys_trun, env_trun, old_eb, old_psi, new_keys):
+    """wave_trans_RL (dmrg_sub.f90:1637-1789): the right-to-left move; the
+    old enlarged env truncated by env_trun becomes the new env block, the
+    old sys site the new env site, and the old sys block is expanded back
+    by sys_trun; both truncations applied without conjugation."""
+    o_sys, o_env = old_eb
+    new = {key: np.zeros((sys_eb.basis[key[0]][2], env_eb.basis[key[1]][2]), complex) for key in new_keys}
+    for (x, y), v in old_psi.items():
+        ns, js = _qn(o_sys, x)
+        ne, je = _qn(o_env, y)
+        ue = env_trun.get((ne, je))
+        if ue is None:
+            continue
+        for (sbl, sst, p, sd) in _subs(o_sys, x):
+            us = sys_trun.get(sbl)
+            if us is None:
+                continue
+            for jn in range(abs(je - sst[1]), je + sst[1] + 1, 2):
+                ye = _sector(env_eb, ne + sst[0], jn)
+                xs = _sector(sys_eb, sbl[0], sbl[1])
+                f = _find(env_eb, ye, (ne, je), sst)
+                if f is None or xs is None or (xs, ye) not in new:
+                    continue
+                q, nd = f
+                j12, j3, j4, j123, j34 = je, sst[1], sbl[1], jn, js
+                c = _pm(j34 + j4 + j3) / np.sqrt((1.0 + j12) * (1.0 + j4))
+                c *= np.sqrt((1.0 + j123) * (1.0 + j34))
+                c *= _pm(j4 + j3 + j12) * _pm(j12 + j3 - j123)
+                sub = c * (us @ v[p:p + sd, :])
+                new[(xs, ye)][:, q:q + nd] += sub @ ue
+    nrm = np.sqrt(sum(np.vdot(m, m).real for m in new.values()))
+    return {k: m / nrm for k, m in new.items()}

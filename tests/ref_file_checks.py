@@ -55,7 +55,10 @@ def grow_side(path, side, upto):
         r = persist.RecordReader(os.path.join(path, persist.trun_name(side, L)))
         head, subs = raw_total_block(r)
         assert head[0] == L
-        U = [(eb.index[(n, j2)], m) for (n, j2, _dd, m) in subs]
+        # the reference's env block states are the conjugated columns of its
+        # env truncation (wave_trans_LR applies it conjugate-transposed,
+        # wave_trans_RL plainly; its kernels conjugate env operators)
+        U = [(eb.index[(n, j2)], np.conj(m) if side == "env" else m) for (n, j2, _dd, m) in subs]
         states[L] = eb.truncate(U)
     return states
 
@@ -84,7 +87,10 @@ def check_block_files(path, side, Lx, Ny, couplings, upto):
         inner = [(local[a], local[b], t, J) for (a, b, t, J) in bl if a in local and b in local]
         want, tens = RB.reduced(st, st, 0, RB.block_hamiltonian(F, inner), 0)
         assert tens < 1e-10
-        assert agree(as_blocks(hsubs, st, 0), want) < 1e-10, ("ham", side, L)
+        # env operators are stored as the complex conjugates of the physical
+        # reduced matrices (the kernels apply them conjugate-transposed)
+        conj = (lambda d: {k: np.conj(v) for k, v in d.items()}) if side == "env" else (lambda d: d)
+        assert agree(as_blocks(hsubs, st, 0), conj(want)) < 1e-10, ("ham", side, L)
         for II in slots[L]:
             site = II if side == "sys" else N + 1 - II
             loc = local[site]
@@ -93,7 +99,7 @@ def check_block_files(path, side, Lx, Ny, couplings, upto):
                 assert head[0] == L and head[3:] == [dN, k], (name, head)
                 want, tens = RB.reduced(st, st, k, ops[loc][name][1], dN)
                 assert tens < 1e-10, (name, L)
-                assert agree(as_blocks(subs, st, dN), want) < 1e-10, (side, L, site, name)
+                assert agree(as_blocks(subs, st, dN), conj(want)) < 1e-10, (side, L, site, name)
         assert r.pos == len(r.data)
         if side == "env" and L >= 3:
             eb = states[L - 1].enlarge(site_first=True)

@@ -185,3 +185,61 @@ def test_superblock_coupling_order_is_the_references():
             assert lam == pytest.approx(-lam_b, rel=1e-12)
         else:
             assert lam == pytest.approx(lam_b, rel=1e-12)
+
+
+def test_complex_data_through_the_writers(tmp_path):
+    """With complex truncations and a complex wave function (which a real
+    Hamiltonian never produces) the writers still hand the reference the
+    same states: an env truncation written by truns_to_disk and read the
+    reference's way (env block states = conjugated columns, chain-ordered
+    products; fixed by its wave_trans routines, test_ref_kernels.py) spans
+    the same Fock vectors as this build's own env block (block-then-site
+    products), and likewise for a wave function through wave_to_disk."""
+    from tsc_tri_t1t2j1j2_amd.blocks import SuperBasis, TotalBasis
+    from tsc_tri_t1t2j1j2_amd.wavetrans import Trunc
+    rng = np.random.default_rng(11)
+    site = RB.BlockStates.site()
+    nat = site.enlarge(True)                 # env block {6} + site 5, chain order (site first)
+    bs = TotalBasis(list(SITE_SECTORS))
+    assert [(n, j, d) for (n, j, d, _s) in bs.sectors] == nat.sectors
+    # this build's enlarged env states: block modes before the site's, which
+    # is the chain-ordered product times (-1)^(N_block N_site)
+    sign = [C.odd_odd_sign(nat, x) for x in range(len(nat.sectors))]
+
+    def ours(U):
+        return nat.truncate([(x, sign[x][:, None] * U[x]) for x in sorted(U)])
+
+    U = {}
+    for x, (_n, _j, d) in enumerate(nat.sectors):
+        q, _ = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+        U[x] = q[:, :max(1, d - 1)]
+    p = str(tmp_path / "envtruns2")
+    persist.truns_to_disk(p, Trunc(bs, U, sorted(U)), 2, env=True)
+    head, subs = C.raw_total_block(persist.RecordReader(p))
+    theirs = nat.truncate([(nat.index[(n, j2)], np.conj(m)) for (n, j2, _dd, m) in subs])
+    mine = ours(U)
+    for x in range(len(mine.sectors)):
+        for c in range(mine.sectors[x][2]):
+            for M, v in mine.vecs[x][c].items():
+                assert np.abs(theirs.vecs[x][c][M] - v).max() < 1e-14
+    # a complex wave function over (sys {1} + site 2) x (site 3 + env {4})
+    sys_nat = site.enlarge(False)
+    env_nat = nat
+    sb = SuperBasis(bs, bs, 2, 0)
+    psi = rng.normal(size=sb.dim) + 1j * rng.normal(size=sb.dim)
+    psi /= np.linalg.norm(psi)
+    persist.wave_to_disk(str(tmp_path / "eigvec1"), sb, bs, bs, psi, 2, 2)
+    r = persist.RecordReader(str(tmp_path / "eigvec1"))
+    _sl, _el, num, _dim = (r.int1() for _ in range(4))
+    stored = []
+    for _ in range(num):
+        r.empty()
+        sq, eq = tuple(r.ints(2)), tuple(r.ints(2))
+        dx, dy = r.ints(2)
+        stored.append((sys_nat.index[sq], env_nat.index[eq], r.matrix(dx, dy)))
+    theirs = RB.super_state(sys_nat, env_nat, stored)
+    mine = 0.0
+    for i, (x, y, dx, dy) in enumerate(sb.sub):
+        m = psi[sb.offset[i]:sb.offset[i] + dx * dy].reshape((dx, dy), order="F") * sign[y][None, :]
+        mine = mine + RB.super_state(sys_nat, env_nat, [(x, y, m)])
+    assert np.abs(theirs - mine).max() < 1e-14

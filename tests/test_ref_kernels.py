@@ -187,3 +187,53 @@ def test_a_sign_slip_is_seen(sb):
         M = _matrix(sb, fn)
         lam = (np.vdot(XM, M) / np.vdot(XM, XM)).real
         assert np.abs(M - lam * XM).max() > 1e-3
+
+
+def test_wave_trans_keeps_the_state():
+    """wave_trans_LR / wave_trans_RL (dmrg_sub.f90:1471-1789), restated, carry
+    a wave function to the next superblock of a sweep without changing the
+    physical state when the truncations are complete, here random complex
+    unitaries per sector: the Fock vector before and after agree to 1e-15.
+    This fixes the recoupling phases of both routines and that the
+    reference's env block states are the conjugated columns of its env
+    truncation matrices (reading them as plain columns fails)."""
+    rng = np.random.default_rng(7)
+
+    def unitaries(eb):
+        out = {}
+        for (n, j, d) in eb.sectors:
+            q, _ = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+            out[(n, j)] = q
+        return out
+
+    def apply(eb, U, env):
+        return eb.truncate([(eb.index[k], np.conj(u) if env else u) for k, u in U.items()])
+
+    def keys_of(se, ee):
+        return [(x, y) for x, bx in enumerate(se.basis) for y, by in enumerate(ee.basis)
+                if bx[0] + by[0] == 4 and bx[1] == by[1]]
+
+    def state(se, ee, psi):
+        return RB.super_state(se, ee, [(x, y, m) for (x, y), m in psi.items()])
+
+    site = RB.BlockStates.site()
+    s12, e56 = site.enlarge(False), site.enlarge(True)
+    Us2, Ue2 = unitaries(s12), unitaries(e56)
+    so, eo = apply(s12, Us2, False).enlarge(False), apply(e56, Ue2, True).enlarge(True)   # 1-3 | 4-6
+    psi = {k: rng.normal(size=(so.basis[k[0]][2], eo.basis[k[1]][2]))
+           + 1j * rng.normal(size=(so.basis[k[0]][2], eo.basis[k[1]][2])) for k in keys_of(so, eo)}
+    nrm = np.sqrt(sum(np.vdot(m, m).real for m in psi.values()))
+    psi = {k: m / nrm for k, m in psi.items()}
+    before = state(so, eo, psi)
+    # left to right: sys 1-3 truncated (complete), new sys site 4, env 5-6 expanded
+    Us3 = unitaries(so)
+    sn = apply(so, Us3, False).enlarge(False)
+    new = K.wave_trans_LR(sn, e56, Us3, Ue2, (so, eo), psi, keys_of(sn, e56))
+    assert np.linalg.norm(state(sn, e56, new) - before) < 1e-13
+    # right to left: env 4-6 truncated, new env site 3, sys 1-2 expanded
+    Ue3 = unitaries(eo)
+    en = apply(eo, Ue3, True).enlarge(True)
+    new = K.wave_trans_RL(s12, en, Us2, Ue3, (so, eo), psi, keys_of(s12, en))
+    assert np.linalg.norm(state(s12, en, new) - before) < 1e-13
+    wrong = apply(eo, Ue3, False).enlarge(True)
+    assert np.linalg.norm(state(s12, wrong, new) - before) > 0.1
