@@ -48,5 +48,9 @@ parameters):
                       a dropped block_transfer_trans or env sign breaks it;
                       wave_trans_LR/RL keep a complex random state exactly
                       under complete complex unitary truncations, and fail
-                      when env block states are read as unconjugated columns
+                      when env block states are read as unconjugated columns;
+                      the six enlargement routines give the physical reduced
+                      matrices in the enlarged basis on both sides, and the
+                      block-site bond terms of Get_Hamiltonian_Sys/Env sum
+                      to the physical bond
 """

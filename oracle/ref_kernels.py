@@ -530,3 +530,163 @@ def wave_trans_RL(sys_eb, env_eb, sys_trun, env_trun, old_eb, old_psi, new_keys)
                 new[(xs, ye)][:, q:q + nd] += sub @ ue
     nrm = np.sqrt(sum(np.vdot(m, m).real for m in new.values()))
     return {k: m / nrm for k, m in new.items()}
+
+
+# ---------------------------------------------------------------------
+# enlargement: an operator of the block or of the new site carried into
+# the enlarged basis (block.f90:2495-3459)
+
+def _enlarged_op(eb, up, k, place):
+    """Total_Block skeleton of an enlarged-basis operator: every (ket
+    sector, dd) whose bra sector exists (update_block_ndia_sys's first two
+    loops), with zero matrices; place(ket x, bra x, dd, M) fills one."""
+    subs = {}
+    for x, (n, j, d, _s) in enumerate(eb.basis):
+        for dd in range(-k, k + 1, 2):
+            xb = _sector(eb, n + up, j + dd)
+            if xb is None:
+                continue
+            M = np.zeros((eb.basis[xb][2], d), complex)
+            place(x, xb, dd, M)
+            subs[(n, j, dd)] = M
+    return ROp(up, k, subs)
+
+
+def update_block_ndia(block, eb, flag, env):
+    """update_block_ndia_sys (block.f90:2624-2820) and, env = True,
+    update_block_ndia_env (block.f90:2828-3036): a block operator in the
+    enlarged basis; the env routine signs fermionic operators by
+    (-1)^N_site of the ket's site."""
+    k, up = block.k, block.up
+
+    def place(x, xb, dd, M):
+        _n, J, _d, _s = eb.basis[x]
+        Jb = eb.basis[xb][1]
+        for (bl, st, p, sd) in _subs(eb, x):
+            for sdd in range(-k, k + 1, 2):
+                m = block.get(bl[0], bl[1], sdd)
+                if m is None:
+                    continue
+                new_bl = (bl[0] + up, bl[1] + sdd)
+                f = _find(eb, xb, new_bl, st)
+                if f is None:
+                    continue
+                q, nd = f
+                c = w6j(Jb, k, J, bl[1], st[1], new_bl[1]) * np.sqrt((1.0 + Jb) * (1.0 + J))
+                c *= _pm(new_bl[1] + st[1] + J + k)
+                if env and flag == "F":
+                    c *= _parity(st[0])
+                M[q:q + nd, p:p + sd] += c * m
+
+    return _enlarged_op(eb, up, k, place)
+
+
+def update_site_ndia(site, eb, flag, env):
+    """update_site_ndia_sys (block.f90:3047-3260), signing fermionic site
+    operators by (-1)^N_block, and update_site_ndia_env (block.f90:
+    3269-3459), which signs nothing."""
+    k, up = site.k, site.up
+
+    def place(x, xb, dd, M):
+        _n, J, _d, _s = eb.basis[x]
+        Jb = eb.basis[xb][1]
+        for (bl, st, p, sd) in _subs(eb, x):
+            for sdd in range(-k, k + 1, 2):
+                m = site.get(st[0], st[1], sdd)
+                if m is None:
+                    continue
+                new_st = (st[0] + up, st[1] + sdd)
+                f = _find(eb, xb, bl, new_st)
+                if f is None:
+                    continue
+                q, _nd = f
+                c = w6j(Jb, k, J, st[1], bl[1], new_st[1]) * _pm(bl[1] + st[1] + Jb + k)
+                c *= np.sqrt((1.0 + Jb) * (1.0 + J))
+                if not env and flag == "F":
+                    c *= _parity(bl[0])
+                M[q:q + sd, p:p + sd] += c * m[0, 0] * np.eye(sd)
+
+    return _enlarged_op(eb, up, k, place)
+
+
+def update_block_dia(block, eb):
+    """update_block_dia (block.f90:2495-2548): a block scalar (reduced
+    elements) in the enlarged basis, times sqrt((2J+1)/(2j_block+1))."""
+    def place(x, xb, dd, M):
+        J = eb.basis[x][1]
+        for (bl, st, p, sd) in _subs(eb, x):
+            m = block.get(bl[0], bl[1], 0)
+            if m is not None:
+                M[p:p + sd, p:p + sd] += np.sqrt((1.0 + J) / (1.0 + bl[1])) * m
+
+    return _enlarged_op(eb, 0, 0, place)
+
+
+def update_site_dia(site, eb):
+    """update_site_dia (block.f90:2554-2615)."""
+    def place(x, xb, dd, M):
+        J = eb.basis[x][1]
+        for (bl, st, p, sd) in _subs(eb, x):
+            m = site.get(st[0], st[1], 0)
+            if m is not None:
+                M[p:p + sd, p:p + sd] += np.sqrt((1.0 + J) / (1.0 + st[1])) * m[0, 0] * np.eye(sd)
+
+    return _enlarged_op(eb, 0, 0, place)
+
+
+# ---------------------------------------------------------------------
+# the enlarged block's Hamiltonian (Get_Hamiltonian_Sys/Env,
+# get_sys.f90:4-592)
+
+def get_hamiltonian_ndia(block, site, eb, coef, flag_sign, flags):
+    """Get_hamiltonian_ndia (get_sys.f90:865-1056): the scalar coupling of
+    a block operator and a site operator (site data read as <st|.|new st>,
+    new st = st - up_dif of the site data) added to the enlarged block's
+    Hamiltonian, with its Hermitian conjugate when the particle number
+    changes; flags 'S' (sys: fermion sign (-1)^N of the ket's block) or
+    'E' (env: -1 when the ket's site holds an even number). Returns
+    {enlarged sector: matrix}."""
+    H = {x: np.zeros((d, d), complex) for x, (_n, _j, d, _s) in enumerate(eb.basis)}
+    up, k = block.up, block.k
+    for x, (_n, J, _d, _s) in enumerate(eb.basis):
+        for (bl, st, p, sd) in _subs(eb, x):
+            for sdd in range(-k, k + 1, 2):
+                m = block.get(bl[0], bl[1], sdd)
+                if m is None:
+                    continue
+                new_bl = (bl[0] + up, bl[1] + sdd)
+                for jn in range(abs(st[1] - site.k), st[1] + site.k + 1, 2):
+                    new_st = (st[0] - site.up, jn)
+                    sm = site.get(new_st[0], new_st[1], st[1] - jn)
+                    if sm is None:
+                        continue
+                    f = _find(eb, x, new_bl, new_st)
+                    if f is None:
+                        continue
+                    q, nd = f
+                    c = w6j(new_bl[1], bl[1], k, st[1], new_st[1], J)
+                    c *= np.sqrt((1.0 + J) / (1.0 + k)) * _pm(bl[1] + new_st[1] + k + J) * sm[0, 0]
+                    sign = 1.0
+                    if flag_sign == "F":
+                        if flags == "S":
+                            sign = _parity(bl[0])
+                        elif st[0] % 2 == 0:
+                            sign = -1.0
+                    c1 = c * coef * sign
+                    if c1 == 0.0:
+                        continue
+                    H[x][q:q + nd, p:p + sd] += c1 * m
+                    if site.up != 0:
+                        H[x][p:p + sd, q:q + nd] += np.conj(c1) * np.conj(m).T
+    return H
+
+
+def get_hamiltonian_dia(A, B, eb, coef):
+    """Get_hamiltonian_dia (get_sys.f90:776-854) with flags 'N', 'N' and no
+    conjugate part: coef / sqrt(2J+1) A B per enlarged sector (A, B scalars
+    in the enlarged basis, as update_block_dia / update_site_dia give)."""
+    H = {}
+    for x, (n, J, d, _s) in enumerate(eb.basis):
+        a, b = A.get(n, J, 0), B.get(n, J, 0)
+        H[x] = np.zeros((d, d), complex) if a is None or b is None else coef / np.sqrt(1.0 + J) * (a @ b)
+    return H

@@ -237,3 +237,108 @@ def test_wave_trans_keeps_the_state():
     assert np.linalg.norm(state(s12, en, new) - before) < 1e-13
     wrong = apply(eo, Ue3, False).enlarge(True)
     assert np.linalg.norm(state(s12, wrong, new) - before) > 0.1
+
+
+@pytest.mark.parametrize("side", ["sys", "env"])
+def test_enlargement_routines_give_the_physical_operators(side):
+    """update_block_ndia_sys/env, update_site_ndia_sys/env, update_block_dia
+    and update_site_dia (block.f90:2495-3459), restated, carry the block's
+    and the new site's c+, c~, S and n into the enlarged basis as exactly
+    the reduced matrices of the physical operators between the enlarged
+    states (sys: block then site in chain order; env: site then block), the
+    6j coefficients and both sides' sign rules included; swapping the sign
+    rules of the two sides breaks it."""
+    env = side == "env"
+    site = RB.BlockStates.site()
+
+    def whole(eb):
+        return eb.truncate([(x, np.eye(d)) for x, (_, _, d) in enumerate(eb.sectors)])
+
+    blk = whole(site.enlarge(env))                       # two sites
+    eb = blk.enlarge(env)                                # three sites
+    _F2, o2 = RB.fock_ops(2)
+    _F3, o3 = RB.fock_ops(3)
+    # chain-order positions in the 3-site Fock space: sys block 0, 1 + site 2;
+    # env site 0 + block 1, 2
+    blk_loc = {0: 0, 1: 1} if not env else {0: 1, 1: 2}
+    new_loc = 2 if not env else 0
+    checked = 0
+    for name, (fock, up, k) in NAMES.items():
+        flag = "F" if up else "B"
+        for b, loc in blk_loc.items():
+            red, dev = RB.reduced(blk, blk, k, o2[b][fock][1], up)
+            assert dev < 1e-12
+            op = K.from_reduced(red, blk.sectors, up, k)
+            got = K.update_block_dia(op, eb) if k == 0 else K.update_block_ndia(op, eb, flag, env)
+            want, dev = RB.reduced(eb, eb, k, o3[loc][fock][1], up)
+            assert dev < 1e-12
+            want = K.from_reduced(want, eb.sectors, up, k)
+            for key in set(got.subs) | set(want.subs):
+                a, w = got.subs.get(key), want.subs.get(key)
+                a = np.zeros_like(w) if a is None else a
+                w = np.zeros_like(a) if w is None else w
+                assert np.abs(a - w).max() < 1e-12, (side, name, b, key)
+            checked += 1
+        sop = K.site_op({"c+": "st_elec_up", "c~": "st_elec_down", "S": "st_sd", "n": "num_elec"}[fock])
+        got = K.update_site_dia(sop, eb) if k == 0 else K.update_site_ndia(sop, eb, flag, env)
+        want, dev = RB.reduced(eb, eb, k, o3[new_loc][fock][1], up)
+        want = K.from_reduced(want, eb.sectors, up, k)
+        for key in set(got.subs) | set(want.subs):
+            a, w = got.subs.get(key), want.subs.get(key)
+            a = np.zeros_like(w) if a is None else a
+            w = np.zeros_like(a) if w is None else w
+            assert np.abs(a - w).max() < 1e-12, (side, name, "site", key)
+        # the other side's sign rule is wrong here
+        if flag == "F":
+            bad = K.update_site_ndia(sop, eb, flag, not env)
+            assert max(np.abs(bad.subs[key] - want.subs.get(key, 0 * bad.subs[key])).max() for key in bad.subs) > 0.1
+        checked += 1
+    assert checked == 12
+
+
+@pytest.mark.parametrize("side", ["sys", "env"])
+def test_block_site_bond_of_the_enlarged_hamiltonian(side):
+    """Get_Hamiltonian_Sys/Env's bond terms (get_sys.f90:40-180, 331-460):
+    Get_hamiltonian_ndia with jt on (elec_up, trans(st_elec_down)) for the
+    sys side or (elec_down, trans(st_elec_up)) for the env side, with jd on
+    (spin_sd, st_sd), and Get_hamiltonian_dia with jn on the enlarged n's,
+    restated, sum to the reduced matrix of the physical bond
+    -t sum_s (c+ c + h.c.) + J (S.S - n n / 4) between a block site and the
+    new site, for the couplings main.f90:61-73 sets (jt = -sqrt2 t,
+    jd = -sqrt3 J, jn = -J/4)."""
+    env = side == "env"
+    site = RB.BlockStates.site()
+
+    def whole(eb):
+        return eb.truncate([(x, np.eye(d)) for x, (_, _, d) in enumerate(eb.sectors)])
+
+    blk = whole(site.enlarge(env))
+    eb = blk.enlarge(env)
+    _F2, o2 = RB.fock_ops(2)
+    F3 = RB.R.Fock(3)
+    new_loc = 0 if env else 2
+    T = K.trans
+    for b in (0, 1):
+        loc = b + 1 if env else b
+
+        def op(fock, up, k):
+            red, dev = RB.reduced(blk, blk, k, o2[b][fock][1], up)
+            assert dev < 1e-12
+            return K.from_reduced(red, blk.sectors, up, k)
+
+        for (t, J) in ((1.0, 0.0), (0.0, 1.0), (0.7, 0.3)):
+            jt, jd, jn = -np.sqrt(2.0) * t, -np.sqrt(3.0) * J, -0.25 * J
+            if env:
+                hop = K.get_hamiltonian_ndia(op("c~", -1, 1), T(K.site_op("st_elec_up")), eb, jt, "F", "E")
+            else:
+                hop = K.get_hamiltonian_ndia(op("c+", 1, 1), T(K.site_op("st_elec_down")), eb, jt, "F", "S")
+            spin = K.get_hamiltonian_ndia(op("S", 0, 2), K.site_op("st_sd"), eb, jd, "B", "S" if not env else "E")
+            nn = K.get_hamiltonian_dia(K.update_block_dia(op("n", 0, 0), eb), K.update_site_dia(K.site_op("num_elec"), eb),
+                                       eb, jn)
+            a, c_ = min(loc, new_loc), max(loc, new_loc)
+            X = RB.block_hamiltonian(F3, [(a, c_, t, J)])
+            want, dev = RB.reduced(eb, eb, 0, X, 0)
+            assert dev < 1e-12
+            for x in range(len(eb.basis)):
+                got = hop[x] + spin[x] + nn[x]
+                assert np.abs(got - want.get((x, x), 0 * got)).max() < 1e-12, (side, b, t, J, x)
