@@ -38,7 +38,8 @@ def test_fortran_drop_in_subroutines_reproduce_ed(tmp_path):
     e_ed = tj_ed.ground_energy(N, Ne, b)
     inp = tmp_path / "bonds.txt"
     inp.write_text(f"{N} {Ne} {len(b)}\n" + "".join(f"{i} {j} {t!r} {J!r}\n" for (i, j, t, J) in b))
-    out = subprocess.run([EXE2, str(inp)], capture_output=True, text=True, timeout=120)
+    dump = tmp_path / "blocks.txt"
+    out = subprocess.run([EXE2, str(inp), str(dump)], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     vals = {}
@@ -52,3 +53,54 @@ def test_fortran_drop_in_subroutines_reproduce_ed(tmp_path):
     assert abs(e1 - e_ed) / N < 1e-10, (e1, e_ed)
     assert abs(vals["RQ"][0] - e1) < 1e-9 and vals["RQ"][1] < 1e-12
     assert abs(e2 - e_ed) / N < 1e-10, (e2, e_ed)
+    _blocks_are_the_references(dump.read_text(), N, b)
+
+
+def _blocks_are_the_references(text, N, bond_list):
+    """The untruncated 3-site sys and env blocks the drop-in subroutines
+    built hold the reference's numbers: each sub-block of the Hamiltonian and
+    of every carried site's c+ (elec_up), S (spin_sd), n (num_sn) equals the
+    reduced matrix of the physical operator between the reference's block
+    states (oracle/ref_blocks.py: block (x) site coupling, chain fermion
+    order, identity truncations), conjugated on the env side as the
+    reference stores env operators."""
+    import numpy as np
+    pytest.importorskip("sympy")
+    from oracle import ref_blocks as RB
+    subs = {"sys": [], "env": []}
+    for line in text.splitlines():
+        f = line.split()
+        if f and f[0] == "SUB":
+            side, name, site, up = f[1], f[2], int(f[3]), int(f[4])
+            nu, nd, dd, rows, cols = (int(v) for v in f[5:10])
+            vals = np.array([float(v) for v in f[10:]])
+            m = (vals[0::2] + 1j * vals[1::2]).reshape((rows, cols), order="F")
+            subs[side].append((name, site, up, nu, nd, dd, m))
+    seen = 0
+    for side in ("sys", "env"):
+        env = side == "env"
+        st = RB.BlockStates.site()
+        for _ in range(2):
+            eb = st.enlarge(env)
+            st = eb.truncate([(x, np.eye(d)) for x, (_, _, d) in enumerate(eb.sectors)])
+        sites = [1, 2, 3] if not env else [N - 2, N - 1, N]
+        local = {s: k for k, s in enumerate(sites)}
+        F, ops = RB.fock_ops(3)
+        inner = [(local[a], local[c], t, J) for (a, c, t, J) in bond_list if a in local and c in local]
+        want_of = {"H": (0, 0, RB.block_hamiltonian(F, inner))}
+        for (name, site, up, nu, nd, dd, m) in subs[side]:
+            if name == "H":
+                k, dN, comps = want_of["H"]
+            else:
+                fock, dN, k = {"c": ("c+", 1, 1), "S": ("S", 0, 2), "n": ("n", 0, 0)}[name]
+                comps = ops[local[site]][fock][1]
+            assert up == dN
+            want, dev = RB.reduced(st, st, k, comps, dN)
+            assert dev < 1e-12
+            key = (st.index[(nu + up, nd + dd)], st.index[(nu, nd)])
+            w = want.get(key, np.zeros_like(m))
+            if env:
+                w = np.conj(w)
+            assert np.abs(m - w).max() < 1e-12, (side, name, site, nu, nd, dd)
+            seen += 1
+    assert seen > 40

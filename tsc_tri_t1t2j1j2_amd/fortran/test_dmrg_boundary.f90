@@ -5,7 +5,10 @@
 ! (Get_density_matrix_sys, Get_truncation_operator keeping every state,
 ! Get_Hamiltonian_Sys) grows the sys block once more for a second
 ! superblock. Reads "N Ne" and then one bond "i j t J" per line from the
-! file named on the command line; prints the numbers the test compares.
+! file named on the command line; prints the numbers the test compares and,
+! when a second file name is given, writes the untruncated sys and env
+! blocks there (every sub-block of the Hamiltonian and of each carried
+! site's c+, S, n) for a comparison with the reference's own numbers.
 program test_dmrg_boundary
   use iso_c_binding
   use tsc_hotpath
@@ -50,6 +53,13 @@ program test_dmrg_boundary
     call Get_Hamiltonian_Env(env, nxt, ebs, trun)
   end do
   write(*, '(a,3i6)') 'LENS ', sys%len, env%len, sys%ham%num
+  if (command_argument_count() >= 2) then
+    call get_command_argument(2, path)
+    open(newunit=u, file=trim(path), status='replace', action='write')
+    call dump_model(u, 'sys', sys)
+    call dump_model(u, 'env', env)
+    close(u)
+  end if
 
   ! update_trun_dia with every state kept returns the operator unchanged
   call Get_basis(sys%ham, sbs)
@@ -107,6 +117,40 @@ program test_dmrg_boundary
   write(*, '(a,i10,f24.15)') 'E2 ', super%dim, Energys(1)
 
 contains
+
+  ! one line per sub-block: tag, operator, num_up, num_down, down_dif,
+  ! row_dim, sdim, then the matrix column-major as real, imaginary pairs
+  subroutine dump_model(u, tag, m)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: tag
+    type(Total_Model), intent(in) :: m
+    integer :: k
+    write(u, '(a,1x,i4,1x,i4)') 'MODEL '//tag, m%len, size(m%keep)
+    write(u, '(a,1x,100i4)') 'KEEP '//tag, m%keep
+    call dump_block(u, tag, 'H', 0, m%ham)
+    do k = 1, size(m%keep)
+      call dump_block(u, tag, 'c', m%keep(k), m%sub(k)%elec_up)
+      call dump_block(u, tag, 'S', m%keep(k), m%sub(k)%spin_sd)
+      call dump_block(u, tag, 'n', m%keep(k), m%sub(k)%num_sn)
+    end do
+  end subroutine dump_model
+
+  subroutine dump_block(u, tag, name, site, b)
+    integer, intent(in) :: u, site
+    character(len=*), intent(in) :: tag, name
+    type(Total_Block), intent(in) :: b
+    integer :: q, i, j
+    do q = 1, b%num
+      write(u, '(a,1x,a,1x,i4,1x,i4,6(1x,i4))', advance='no') 'SUB '//tag, name, site, b%up_dif, &
+        b%sub(q)%num_up, b%sub(q)%num_down, b%sub(q)%down_dif, b%sub(q)%row_dim, b%sub(q)%sdim
+      do j = 1, b%sub(q)%sdim
+        do i = 1, b%sub(q)%row_dim
+          write(u, '(2(1x,es24.16))', advance='no') real(b%sub(q)%mat(i, j)), aimag(b%sub(q)%mat(i, j))
+        end do
+      end do
+      write(u, *)
+    end do
+  end subroutine dump_block
 
   ! the identity truncation restricted to the block's own sectors
   subroutine sector_trun(ham, t)

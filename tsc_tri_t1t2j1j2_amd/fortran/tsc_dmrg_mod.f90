@@ -27,11 +27,19 @@
 ! of its Hamiltonian, in order (every sector has a diagonal block). An
 ! operator's sub(k) is keyed by its ket sector (num_up, num_down) and
 ! down_dif; the bra sector is (num_up + up_dif, num_down + down_dif);
-! mat(row_dim = bra dim, sdim = ket dim). Env-block operators are stored as
-! this library applies them, psi . O^T, the complex conjugates of the
-! reference's (which applies psi . O^H, get_sys.f90:1869); Get_Hamiltonian_Env
-! produces them that way from the reference's rho_env eigenvectors. The
-! model carries, for every site with a bond leaving the block, that site's
+! mat(row_dim = bra dim, sdim = ket dim). The types hold the reference's
+! numbers, env side included: env-block operators as the reference stores
+! them (it applies them as psi . O^H, get_sys.f90:1869), env truncations as
+! its rho_env eigenvectors, and wave functions whose enlarged env basis puts
+! the site's fermion modes before the block's, as its
+! update_block_ndia_env / update_site_ndia_env signs imply
+! (block.f90:2828-3459). The library applies env operators as psi . O^T and
+! orders an enlarged env block then site, so the boundary converts: env
+! operators are conjugated when packed and when stored, env truncation
+! matrices are conjugated and their rows, like the env columns of a wave
+! function, multiplied by (-1)^(N_block N_site) of their enlarged-env
+! product state (env_signs). The model carries, for every site with a bond
+! leaving the block, that site's
 ! c+ (elec_up), S (spin_sd) and n (num_sn); `sites` lists the block's sites
 ! and `keep` the carried ones. Bonds (i, j, t, J) are set once with
 ! dmrg_setup_bonds (the role of dmrg_setup's neighbour tables,
@@ -121,7 +129,8 @@ module tsc_dmrg
     type(tsc_model_desc) :: d
   end type ModelPack
 
-  private :: pack_model, arena_add, dev, sector_of, flatten_wave, unflatten_wave, superblock_plan, find_trun
+  private :: pack_model, arena_add, dev, sector_of, flatten_wave, unflatten_wave, superblock_plan, find_trun, &
+             env_signs, env_sector
 
 contains
 
@@ -421,11 +430,13 @@ contains
     end do
   end function sector_of
 
-  ! C descriptors of a block model; its matrices appended to the arena.
-  ! side 1 (env): operators are stored as the library applies them.
-  subroutine pack_model(m, side, ar, n, pk)
+  ! C descriptors of a block model; its matrices appended to the arena
+  ! (conjugated for an env block, whose operators the types hold as the
+  ! reference stores them).
+  subroutine pack_model(m, side, ar, n, pk, conj)
     type(Total_Model), intent(in) :: m
     integer, intent(in) :: side
+    logical, intent(in) :: conj
     complex(c_double_complex), allocatable, intent(inout) :: ar(:)
     integer(8), intent(inout) :: n
     type(ModelPack), target, intent(inout) :: pk
@@ -468,34 +479,74 @@ contains
         bra = sector_of(m%ham, b%sub(q)%num_up + b%up_dif, b%sub(q)%num_down + b%sub(q)%down_dif)
         if (ket < 0 .or. bra < 0) error stop 'operator sub-block outside the block sectors'
         pk%ops(slot)%b(q) = tsc_op_block(int(bra, c_int32_t), int(ket, c_int32_t), int(n, c_int64_t))
-        call arena_add(ar, n, b%sub(q)%mat, .false.)
+        call arena_add(ar, n, b%sub(q)%mat, conj)
       end do
       pk%desc(slot) = tsc_op_desc(int(side, c_int32_t), int(up_dif, c_int32_t), int(k2, c_int32_t), &
                                   int(b%num, c_int32_t), c_loc(pk%ops(slot)%b))
     end subroutine op
   end subroutine pack_model
 
-  subroutine flatten_wave(w, flat)
+  ! (-1)^(N_block N_site) over the rows of sector x of an enlarged env
+  ! basis: -1 where the product state has an odd block and an odd site
+  subroutine env_signs(basis, x, sg)
+    type(Total_Basis), intent(in) :: basis
+    integer, intent(in) :: x
+    real(8), allocatable, intent(out) :: sg(:)
+    integer :: j, p
+    allocate(sg(max(basis%sub(x)%dim, 1)))
+    sg = 1.0d0
+    do j = 1, basis%sub(x)%num
+      if (mod(basis%sub(x)%sub(j)%bl_num_up, 2) /= 0 .and. mod(basis%sub(x)%sub(j)%st_num_up, 2) /= 0) then
+        p = basis%sub(x)%sub(j)%spos
+        sg(p + 1:p + basis%sub(x)%sub(j)%sdim) = -1.0d0
+      end if
+    end do
+  end subroutine env_signs
+
+  integer function env_sector(basis, num_up, num_down)
+    type(Total_Basis), intent(in) :: basis
+    integer, intent(in) :: num_up, num_down
+    integer :: x
+    env_sector = 0
+    do x = 1, basis%num
+      if (basis%sub(x)%new_num_up == num_up .and. basis%sub(x)%new_num_down == num_down) env_sector = x
+    end do
+    if (env_sector == 0) error stop 'wave function sector outside the env basis'
+  end function env_sector
+
+  ! the library's layout of a wave function: sectors end to end, env
+  ! columns times env_signs (the same map both ways)
+  subroutine flatten_wave(w, env_bs, flat)
     type(Wavefunction), intent(in) :: w
+    type(Total_Basis), intent(in) :: env_bs
     complex(c_double_complex), intent(out) :: flat(:)
+    real(8), allocatable :: sg(:)
     integer(8) :: o
-    integer :: i
+    integer :: i, c
     o = 0
     do i = 1, w%num
-      flat(o + 1:o + size(w%sub(i)%vec, kind=8)) = reshape(w%sub(i)%vec, [size(w%sub(i)%vec, kind=8)])
-      o = o + size(w%sub(i)%vec, kind=8)
+      call env_signs(env_bs, env_sector(env_bs, w%sub(i)%env_num_up, w%sub(i)%env_num_down), sg)
+      do c = 1, w%sub(i)%env_dim
+        flat(o + 1:o + w%sub(i)%sys_dim) = w%sub(i)%vec(:, c) * sg(c)
+        o = o + w%sub(i)%sys_dim
+      end do
     end do
   end subroutine flatten_wave
 
-  subroutine unflatten_wave(flat, w)
+  subroutine unflatten_wave(flat, env_bs, w)
     complex(c_double_complex), intent(in) :: flat(:)
+    type(Total_Basis), intent(in) :: env_bs
     type(Wavefunction), intent(inout) :: w
+    real(8), allocatable :: sg(:)
     integer(8) :: o
-    integer :: i
+    integer :: i, c
     o = 0
     do i = 1, w%num
-      w%sub(i)%vec = reshape(flat(o + 1:o + size(w%sub(i)%vec, kind=8)), shape(w%sub(i)%vec))
-      o = o + size(w%sub(i)%vec, kind=8)
+      call env_signs(env_bs, env_sector(env_bs, w%sub(i)%env_num_up, w%sub(i)%env_num_down), sg)
+      do c = 1, w%sub(i)%env_dim
+        w%sub(i)%vec(:, c) = flat(o + 1:o + w%sub(i)%sys_dim) * sg(c)
+        o = o + w%sub(i)%sys_dim
+      end do
     end do
   end subroutine unflatten_wave
 
@@ -511,8 +562,8 @@ contains
     type(c_ptr) :: tasks
     integer(c_int64_t), target :: cnt(9)
     n = 0
-    call pack_model(sys, 0, ar, n, ps)
-    call pack_model(env, 1, ar, n, pe)
+    call pack_model(sys, 0, ar, n, ps, .false.)
+    call pack_model(env, 1, ar, n, pe, .true.)
     call tsc_check(tsc_model_tasks(c_loc(ps%d), c_loc(pe%d), int(sys%len + 1, c_int32_t), &
                                    int(num_site - env%len, c_int32_t), int(nbonds(), c_int32_t), c_loc(bonds), &
                                    int(ntot, c_int32_t), int(tot_num_down, c_int32_t), c_loc(ar), &
@@ -548,9 +599,9 @@ contains
     call superblock_plan(sys, env, invec%sub(1)%sys_num_up + invec%sub(1)%env_num_up, plan, dim)
     if (dim /= invec%dim) error stop 'model_wave: superblock dimension differs from the wavefunction'
     allocate(xin(dim), xout(dim))
-    call flatten_wave(invec, xin)
+    call flatten_wave(invec, env_bs, xin)
     call tsc_check(tsc_hpsi_host(plan, c_loc(xin), c_loc(xout), TSC_PREC_F64), 'tsc_hpsi_host')
-    call unflatten_wave(xout, outvec)
+    call unflatten_wave(xout, env_bs, outvec)
     call tsc_check(tsc_plan_destroy(plan), 'tsc_plan_destroy')
   end subroutine model_wave
 
@@ -572,13 +623,13 @@ contains
     call superblock_plan(sys, env, super%sub(1)%sys_num_up + super%sub(1)%env_num_up, plan, dim)
     if (dim /= super%dim) error stop 'lanczos_memo_new: superblock dimension differs from the super basis'
     allocate(x(dim))
-    call flatten_wave(eigvec(1), x)
+    call flatten_wave(eigvec(1), env_bs, x)
     call tsc_check(tsc_vec_create(dev(), int(dim, c_int64_t), psi), 'tsc_vec_create')
     call tsc_check(tsc_vec_upload(psi, c_loc(x)), 'tsc_vec_upload')
     call tsc_check(tsc_lanczos(plan, psi, real(lanzero, c_double), 100_c_int, 4_c_int, lanczos_prec, &
                                lanczos_refine, e, info), 'tsc_lanczos')
     call tsc_check(tsc_vec_download(psi, c_loc(x)), 'tsc_vec_download')
-    call unflatten_wave(x, eigvec(1))
+    call unflatten_wave(x, env_bs, eigvec(1))
     Energys(1) = e
     Energy0 = e
     call tsc_check(tsc_vec_destroy(psi), 'tsc_vec_destroy')
@@ -603,7 +654,13 @@ contains
       nr = nr + int(d, 8) * d
     end do
     allocate(rho(max(nr, 1_8)))
-    call flatten_wave(wave, x)
+    ! rho of the wave function as the types hold it (the reference's
+    ! fermion order): the reference's rho_sys = psi psi^H, rho_env = psi^H psi
+    o = 0
+    do i = 1, wave%num
+      x(o + 1:o + size(wave%sub(i)%vec, kind=8)) = reshape(wave%sub(i)%vec, [size(wave%sub(i)%vec, kind=8)])
+      o = o + size(wave%sub(i)%vec, kind=8)
+    end do
     call tsc_check(tsc_vec_create(dev(), int(wave%dim, c_int64_t), psi), 'tsc_vec_create')
     call tsc_check(tsc_vec_upload(psi, c_loc(x)), 'tsc_vec_upload')
     call tsc_check(tsc_density_matrix(dev(), psi, int(wave%num, c_int32_t), c_loc(sd), c_loc(ed), &
@@ -791,13 +848,14 @@ contains
     integer(c_int64_t), allocatable, target :: offs(:)
     integer(c_int64_t), target :: c3(3), lens(3)
     integer, allocatable :: newsites(:), cnt(:)
+    real(8), allocatable :: sg(:)
     integer(8) :: n, o
     integer :: k, x, q, nk, w, key, b2, k2, nw, ns, s, i
     type(c_ptr) :: tasks, plan
     type(Total_Model) :: nb
     logical :: out
     n = 0
-    call pack_model(pre, 0, ar, n, pk)
+    call pack_model(pre, 0, ar, n, pk, env_side)
     allocate(newsites(pre%len + 1))
     newsites(1:pre%len) = pre%sites(1:pre%len)
     newsites(pre%len + 1) = new_site
@@ -831,13 +889,18 @@ contains
     allocate(uin(max(o, 1_8)))
     o = 0
     do k = 1, trun%num
-      ! env blocks are applied as psi . O^T: their U is conj of rho_env's eigenvectors
       if (env_side) then
-        uin(o + 1:o + size(trun%sub(k)%mat, kind=8)) = reshape(conjg(trun%sub(k)%mat), [size(trun%sub(k)%mat, kind=8)])
+        ! the library's env U: the conjugated eigenvectors of the reference's
+        ! rho_env, rows in its own fermion order (times env_signs)
+        call env_signs(basis, order(k) + 1, sg)
+        do i = 1, trun%sub(k)%sdim
+          uin(o + 1:o + trun%sub(k)%row_dim) = conjg(trun%sub(k)%mat(:, i)) * sg(1:trun%sub(k)%row_dim)
+          o = o + trun%sub(k)%row_dim
+        end do
       else
         uin(o + 1:o + size(trun%sub(k)%mat, kind=8)) = reshape(trun%sub(k)%mat, [size(trun%sub(k)%mat, kind=8)])
+        o = o + size(trun%sub(k)%mat, kind=8)
       end if
-      o = o + size(trun%sub(k)%mat, kind=8)
     end do
     call tsc_check(tsc_renorm_tasks(c_loc(pk%d), int(new_site, c_int32_t), int(nbonds(), c_int32_t), c_loc(bonds), &
                                     int(nk, c_int32_t), c_loc(keep), int(trun%num, c_int32_t), c_loc(order), &
@@ -875,6 +938,9 @@ contains
       k2 = lay(3 * w) + 1
       cnt(key + 1) = cnt(key + 1) + 1
       s = cnt(key + 1)
+      ! an env block's operators are stored as the reference stores them
+      if (env_side) res(offs(w) + 1:offs(w) + int(nsec(3 * b2), 8) * nsec(3 * k2)) = &
+        conjg(res(offs(w) + 1:offs(w) + int(nsec(3 * b2), 8) * nsec(3 * k2)))
       associate (mat => reshape(res(offs(w) + 1:offs(w) + int(nsec(3 * b2), 8) * nsec(3 * k2)), &
                                 [int(nsec(3 * b2)), int(nsec(3 * k2))]))
         if (key < 0) then
