@@ -3,7 +3,10 @@
 // Same sums as the other paths: every contribution C += coef * op(A) op(B)
 // of model_wave's kernels (hyx15903/TSC_Tri_t1t2J1J2 DMRG_code/get_sys.f90:
 // 1817, 1869, 2555-2560, 3385-3408, ...), in complex FP64 with
-// v_mfma_f64_16x16x4f64 (4 real MFMAs per complex 16x16x4 product).
+// v_mfma_f64_16x16x4f64. A complex product takes three real MFMAs instead
+// of four (Gauss's product: with op(A) = ar + i ai, op(B) = br + i bi,
+// P = ar br, Q = ai bi, S = (ar + ai)(br + bi), the product is
+// (P - Q) + i (S - P - Q)); the accumulators hold P, Q and S.
 //
 //  * A workgroup owns one GM x GN output unit (128 x 64); each wave computes
 //    a (16 NG) x 32 part of it as NG x 2 blocks of 16x16.
@@ -109,13 +112,14 @@ __global__ __launch_bounds__(64 * NW, WAVES_PER_SIMD) void gd_kernel(Bufs bufs, 
   // column 16b + l%16 of the wave's part; the row layout of the 16x16x4
   // accumulator is measured once per context (rowmode)
   auto orow = [&](int r) { return rowmode ? (lk + 4 * r) : (4 * lk + r); };
-  d4 aR[NG][2], aI[NG][2];
+  d4 aP[NG][2], aQ[NG][2], aS[NG][2];
 #pragma unroll
   for (int g = 0; g < NG; ++g)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      aR[g][b] = d4{0.0, 0.0, 0.0, 0.0};
-      aI[g][b] = d4{0.0, 0.0, 0.0, 0.0};
+      aP[g][b] = d4{0.0, 0.0, 0.0, 0.0};
+      aQ[g][b] = d4{0.0, 0.0, 0.0, 0.0};
+      aS[g][b] = d4{0.0, 0.0, 0.0, 0.0};
     }
 
   // piece q of a step: q < NPA: A, chunk q/8, row block q%8; else B, chunk
@@ -147,16 +151,15 @@ __global__ __launch_bounds__(64 * NW, WAVES_PER_SIMD) void gd_kernel(Bufs bufs, 
     }
   };
 
-  // With op(A) = ar + i sA ai and op(B) = br + i sB bi (s = -1 for a
-  // conjugated operand): Re += ar br + x1 bi with x1 = -sA sB ai;
-  // Im += x2 bi + x3 br with x2 = sB ar, x3 = sA ai.
+  // op(A) = ar + i sA ai, op(B) = br + i sB bi (s = -1 for a conjugated
+  // operand): P += ar br, Q += (sA ai)(sB bi), S += (ar + sA ai)(br + sB bi)
   auto compute = [&](const tsc_contrib &c, int k0, const Step &L) {
     if (na == 0 || nb == 0) return;
     const int nch = min(NC, (c.k - k0 + 3) >> 2);
     const bool cja = (c.a_op == TSC_OP_C || c.a_op == TSC_OP_CONJ);
     const bool cjb = (c.b_op == TSC_OP_C || c.b_op == TSC_OP_CONJ);
     const uint64_t sgn = 0x8000000000000000ull;
-    const uint64_t m1 = (cja == cjb) ? sgn : 0, m2 = cjb ? sgn : 0, m3 = cja ? sgn : 0;
+    const uint64_t ma = cja ? sgn : 0, mb = cjb ? sgn : 0;
     auto flip = [](double x, uint64_t msk) {
       return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ msk);
     };
@@ -164,43 +167,40 @@ __global__ __launch_bounds__(64 * NW, WAVES_PER_SIMD) void gd_kernel(Bufs bufs, 
     for (int ch = 0; ch < NC; ++ch) {
       if (ch >= nch) break;
       const bool kz = k0 + 4 * ch + lk >= c.k;  // k past the contribution: A is zero
-      double ar[NG], x1[NG], x2[NG], x3[NG];
+      // one B fragment at a time, the A fragments read again for each (LDS
+      // reads are cheap next to FP64 MFMAs; this keeps registers for the
+      // three accumulator sets)
 #pragma unroll
-      for (int g = 0; g < NG; ++g) {
-        double2 v = L[8 * ch + NG * wa + g][lane];
-        if (kz) v = make_double2(0.0, 0.0);
-        ar[g] = v.x;
-        x1[g] = flip(v.y, m1);
-        x2[g] = flip(v.x, m2);
-        x3[g] = flip(v.y, m3);
-      }
-      double2 bv[2];
+      for (int b = 0; b < 2; ++b) {
+        if (b >= nb) break;
+        const double2 bv = L[NPA + NBJ * ch + 2 * wb + b][lane];
+        const double br = bv.x, bi = flip(bv.y, mb), bs = br + bi;
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = L[NPA + NBJ * ch + 2 * wb + b][lane];
-#pragma unroll
-      for (int g = 0; g < NG; ++g) {
-        if (g >= na) break;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          if (b >= nb) break;
-          aR[g][b] = mfma16(ar[g], bv[b].x, aR[g][b]);
-          aI[g][b] = mfma16(x3[g], bv[b].x, aI[g][b]);
-          aR[g][b] = mfma16(x1[g], bv[b].y, aR[g][b]);
-          aI[g][b] = mfma16(x2[g], bv[b].y, aI[g][b]);
+        for (int g = 0; g < NG; ++g) {
+          if (g >= na) break;
+          double2 v = L[8 * ch + NG * wa + g][lane];
+          if (kz) v = make_double2(0.0, 0.0);
+          const double ar = v.x, ai = flip(v.y, ma), as = ar + ai;
+          aP[g][b] = mfma16(ar, br, aP[g][b]);
+          aQ[g][b] = mfma16(ai, bi, aQ[g][b]);
+          aS[g][b] = mfma16(as, bs, aS[g][b]);
         }
       }
     }
   };
 
-  // multiply the accumulators by sr + i si
+  // multiply the accumulated sum by sr + i si: (P, Q, S) becomes
+  // (Re, 0, Re + Im) of the scaled product
   auto rescale = [&](double sr, double si) {
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const d4 x = aR[g][b], y = aI[g][b];
-        aR[g][b] = x * sr - y * si;
-        aI[g][b] = x * si + y * sr;
+        const d4 x = aP[g][b] - aQ[g][b], y = aS[g][b] - aP[g][b] - aQ[g][b];
+        const d4 re = x * sr - y * si, im = x * si + y * sr;
+        aP[g][b] = re;
+        aQ[g][b] = d4{0.0, 0.0, 0.0, 0.0};
+        aS[g][b] = re + im;
       }
   };
   // accumulators hold (sum) / (gr + i gi)
@@ -257,6 +257,17 @@ __global__ __launch_bounds__(64 * NW, WAVES_PER_SIMD) void gd_kernel(Bufs bufs, 
     live = step(lds1, lds0);
   }
   if (gr != 0.0 || gi != 0.0) rescale(gr, gi);
+  // from here on aP holds the real part and aS the imaginary part
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const d4 re = aP[g][b] - aQ[g][b];
+      aS[g][b] = aS[g][b] - aP[g][b] - aQ[g][b];
+      aP[g][b] = re;
+    }
+  d4(&aR)[NG][2] = aP;
+  d4(&aI)[NG][2] = aS;
 
   // elementwise terms (one operand the identity)
   for (int q = t.cbeg; q < t.cend; ++q) {
