@@ -298,6 +298,8 @@ class RealRun:
                "warmup_schedule": f"kept_max {m0}, the last {nr} steps 6000; lanzero {args.warmup_lanzero}",
                "centre_sys_sectors": len(d.sys_blocks[l0].sectors),
                "centre_sys_multiplets": int(sum(x[2] for x in d.sys_blocks[l0].sectors))}
+        if d.precision_record:
+            out["warmup_mixed_precision"] = d.precision_by_phase().get("warmup")
         tol = args.window_lanzero * 15   # the first half-sweep's Lanczos tolerance (main.f90:124-131)
         terr = d.trun_err * 15
         windows = {}
@@ -309,14 +311,24 @@ class RealRun:
             # the guessed steps of the mixed window are also solved in FP64
             # from the same start vectors (same-path error); the first step's
             # FP64 solve is the FP64 window's first step (same start vector)
-            d.shadow, d.shadow_f64 = [], False
+            d.shadow, d.shadow_f64, d.precision_record = [], False, []
             recs = d.centre_window(1 if args.no_window else k, tol, 6000, terr, tag=f"window-{name}",
                                    shadow_from=1 if p != capi.PREC_F64 else None)
-            windows[name] = {"steps": recs, "shadow": list(d.shadow)}
+            windows[name] = {"steps": recs, "shadow": list(d.shadow), "refine": list(d.precision_record)}
             free, _ = ctx.mem_info()
             self.hbm_peak = max(self.hbm_peak, self.free0 - free)
         out["window_tol"] = tol
         mx = windows["mixed"]["steps"]
+        if windows["mixed"]["refine"]:
+            # from each mixed solve's refinement pass (its first product FP64): the mixed Ritz value
+            # against the FP64 <psi|H psi> of its own Ritz vector, per site, and that vector's FP64 residual
+            rr = windows["mixed"]["refine"]
+            out["mixed_ritz_dE_per_site_vs_f64_rayleigh"] = [float(f"{abs(em - ea) / d.N:.3e}")
+                                                             for (_, em, ea, _, _, _) in rr]
+            out["refined_dE_est_per_site"] = [float(f"{wr * abs(em - ea) / d.N:.3e}")
+                                              for (_, em, ea, _, _, wr) in rr]
+            out["mixed_ritz_resid_f64"] = [float(f"{res:.3e}") for (_, _, _, res, _, _) in rr]
+            out["refine_shift_per_site"] = [float(f"{(ea - er) / d.N:.3e}") for (_, _, ea, _, er, _) in rr]
         out["window_steps_mixed"] = [{kk: (round(v, 3) if isinstance(v, float) and kk == "seconds" else v)
                                       for kk, v in r.items()} for r in mx]
         if "f64" in windows:

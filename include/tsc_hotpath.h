@@ -256,6 +256,25 @@ int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_ou
  * Diagnostic; has no counterpart in the reference.
  */
 int tsc_lanczos_replays(const tsc_plan *p, int32_t *replays);
+/*
+ * Precision record of the last tsc_lanczos call on p:
+ *   out[0] the Ritz value of the working-precision pass,
+ *   out[1] <psi|H psi> of its normalised Ritz vector psi, FP64 (the
+ *          refinement pass's first product),
+ *   out[2] |H psi - out[1] psi|, FP64 (the residual norm of that vector),
+ *   out[3] the energy after the refinement pass (what tsc_lanczos returned),
+ *   out[4] the weight of the refined Ritz vector outside psi (1 - z_1^2 of
+ *          the pass's normalised tridiagonal eigenvector).
+ * out[1..4] are NaN when the call ran no refinement pass (FP64 precision or
+ * refine_iters = 0). out[0] - out[1] is the energy error of the mixed
+ * arithmetic at the mixed solution. The refined energy meets the mixed
+ * products of the pass (all but its first) only through the directions
+ * outside psi, so to first order its mixed-arithmetic error is out[4]
+ * times that of a mixed Rayleigh quotient, |out[0] - out[1]| being a
+ * measured sample of the latter. Diagnostic; has no counterpart in the
+ * reference (lanczos_memo_new, eigen.f90:4-199, runs in FP64 only).
+ */
+int tsc_lanczos_refinement(const tsc_plan *p, double *out);  /* out[5] */
 
 /*
  * ---- density matrix, eigensolve, truncation (den_mat.f90) ---------------

@@ -103,7 +103,20 @@ class FakePlan:
     def lanczos(self, psi, tol=1e-10, max_inner=100, max_outer=4, prec=0, refine_iters=0):
         e, v, nmv = olanczos.lanczos(self._mv, psi.x, tol=tol, max_inner=max_inner, max_outer=max_outer)
         psi.x = v
+        # every precision is solved in FP64 here: a refinement pass would find
+        # <psi|H psi> = e; its residual is that of the returned vector
+        nan = float("nan")
+        if prec != capi.PREC_F64 and refine_iters > 0:
+            hv = self._mv(v)
+            self._refine = dict(e_mixed=e, e_rayleigh_f64=float(np.vdot(v, hv).real),
+                                resid_f64=float(np.linalg.norm(hv - np.vdot(v, hv).real * v)), e_refined=e,
+                                w_rest=0.0)
+        else:
+            self._refine = dict(e_mixed=e, e_rayleigh_f64=nan, resid_f64=nan, e_refined=nan, w_rest=nan)
         return e, (nmv, 0)
+
+    def lanczos_refinement(self):
+        return dict(self._refine)
 
     def flops(self):
         return 0.0

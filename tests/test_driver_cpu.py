@@ -68,6 +68,57 @@ def test_shadow_error_by_phase(monkeypatch):
     assert max(ph.values()) < 1e-12
 
 
+def test_mixed_precision_record_by_phase(monkeypatch):
+    """Every mixed solve leaves a refinement record; the per-phase summary
+    covers the warm-up, both half-sweeps and every sweep, and compares the
+    mixed Ritz-energy error with the phase's truncation error. An FP64 run
+    records nothing."""
+    from tsc_tri_t1t2j1j2_amd import capi
+    ctx = fake_device.install(monkeypatch)
+    d = DMRG(ctx, 3, 3, Ne=8, kept_min=4, kept_max=12, sweeps=2, lanzero=1e-10, trun_err=1e-14,
+             native_planner=False, prec=capi.PREC_BF16X3)
+    d.run()
+    assert len(d.precision_record) == len(d.history)
+    ph = d.precision_by_phase()
+    assert set(ph) == {"warmup", "half", "sweep1", "sweep2"}
+    for tag, o in ph.items():
+        # the fake device solves in FP64: <psi|H psi> of its vector is its energy
+        assert o["max_mixed_ritz_dE_per_site"] < 1e-12 and abs(o["max_refine_shift_per_site"]) < 1e-12
+        assert o["max_refined_dE_est_per_site"] == 0.0
+        # (an energy-change stop leaves a residual of about the square root of the tolerance)
+        assert 0.0 <= o["max_resid_f64"] < 1e-2
+        assert o["max_trun_err"] == max(h[3] for h in d.history if h[0] == tag)
+        assert o["dE_below_trun_err"] == (o["max_refined_dE_est_per_site"] <= o["max_trun_err"])
+    assert sum(o["solves"] for o in ph.values()) == len(d.history)
+    d64 = DMRG(ctx, 3, 3, Ne=8, kept_min=4, kept_max=12, sweeps=1, lanzero=1e-10, trun_err=1e-14,
+               native_planner=False, prec=capi.PREC_F64)
+    d64.run()
+    assert d64.precision_record == [] and d64.precision_by_phase() == {}
+
+
+def test_precision_summary_arithmetic():
+    """precision_by_phase on planted records: per-phase maxima of
+    |E_mixed - E_rayleigh| / site and of the residual, the phase's largest
+    truncation error, and the comparison of the two."""
+    d = DMRG.__new__(DMRG)
+    d.N = 10
+    d.history = [("warmup", 2, -1.0, 1e-9, 4, 10), ("warmup", 3, -2.0, 3e-9, 4, 10), ("half", 4, -3.0, 2e-12, 4, 10)]
+    d.precision_record = [(0, -1.0 + 2e-8, -1.0, 1e-5, -1.0 - 4e-9, 1e-2),
+                          (1, -2.0 - 1e-8, -2.0, 3e-5, -2.0 - 1e-9, 0.5),
+                          (2, -3.0 + 5e-10, -3.0, 2e-6, -3.0, 1e-1)]
+    ph = d.precision_by_phase()
+    assert ph["warmup"]["solves"] == 2 and ph["half"]["solves"] == 1
+    assert abs(ph["warmup"]["max_mixed_ritz_dE_per_site"] - 2e-9) < 1e-15
+    # w_rest * raw: 1e-2 * 2e-9 and 0.5 * 1e-9
+    assert abs(ph["warmup"]["max_refined_dE_est_per_site"] - 5e-10) < 1e-16
+    assert abs(ph["warmup"]["max_refine_shift_per_site"] - 4e-10) < 1e-15
+    assert ph["warmup"]["max_resid_f64"] == 3e-5
+    assert ph["warmup"]["max_trun_err"] == 3e-9 and ph["warmup"]["dE_below_trun_err"]
+    assert abs(ph["half"]["max_mixed_ritz_dE_per_site"] - 5e-11) < 1e-16
+    assert abs(ph["half"]["max_refined_dE_est_per_site"] - 5e-12) < 1e-17
+    assert ph["half"]["max_trun_err"] == 2e-12 and not ph["half"]["dE_below_trun_err"]
+
+
 def test_renorm_preparation_failure_reaches_the_caller(monkeypatch):
     """The renormalisation bookkeeping built beside the solve: an error
     raised there ends the step with that error, after the solve."""

@@ -344,6 +344,57 @@ def test_lanczos_matches_exact_and_oracle(ctx, prec):
     assert np.linalg.norm(H @ psi - e * psi) < 1e-4
 
 
+@pytest.mark.parametrize("prec", [capi.PREC_F32, capi.PREC_BF16X3])
+def test_lanczos_refinement_record(ctx, prec):
+    """The precision record of a mixed solve: its working-precision Ritz
+    value, and the FP64 <psi|H psi> and residual norm of that Ritz vector,
+    equal what the dense matrix gives for the vector a solve without
+    refinement returns from the same start; the refined energy is the one
+    returned and lies at or below <psi|H psi> (Rayleigh-Ritz over a space
+    that holds psi). Without a refinement pass only the Ritz value is set."""
+    hs = HermitianSuperblock(dL=[5, 9, 3, 12], dR=[7, 4, 6, 8], npairs=6, seed=3)
+    H = hs.dense()
+    scale = float(np.abs(np.linalg.eigvalsh(H)).max())
+    plan = hs.plan(ctx)
+    rng = np.random.default_rng(5)
+    psi0 = rng.normal(size=hs.n) + 1j * rng.normal(size=hs.n)
+    # one pass of 8 products: the mixed solution is not converged
+    kw = dict(tol=1e-12, max_inner=8, max_outer=0, prec=prec)
+    v = ctx.vec_from(psi0)
+    e_m, _ = plan.lanczos(v, refine_iters=0, **kw)
+    psi_m = v.download()
+    v.close()
+    r0 = plan.lanczos_refinement()
+    assert r0["e_mixed"] == e_m
+    assert all(np.isnan(r0[k]) for k in ("e_rayleigh_f64", "resid_f64", "e_refined", "w_rest"))
+    v = ctx.vec_from(psi0)
+    e_r, _ = plan.lanczos(v, refine_iters=3, **kw)
+    psi_r = v.download()
+    v.close()
+    rec = plan.lanczos_refinement()
+    assert rec["e_mixed"] == e_m   # the same start vector and products
+    x = psi_m / np.linalg.norm(psi_m)
+    hx = H @ x
+    ray = float(np.vdot(x, hx).real)
+    res = float(np.linalg.norm(hx - ray * x))
+    assert res > 1e-4 * scale   # not converged: the residual check is not vacuous
+    assert abs(rec["e_rayleigh_f64"] - ray) < 1e-12 * scale
+    assert abs(rec["resid_f64"] - res) < 1e-9 * scale
+    assert rec["e_refined"] == e_r
+    assert e_r <= rec["e_rayleigh_f64"] + 1e-12 * scale
+    # the weight of the refined vector outside the mixed one
+    assert 0.0 < rec["w_rest"] < 1.0
+    assert abs(rec["w_rest"] - (1.0 - abs(np.vdot(x, psi_r)) ** 2)) < 1e-9
+    # the mixed arithmetic's energy error at its own solution: the
+    # working precision's size, not zero
+    assert 0.0 < abs(rec["e_mixed"] - rec["e_rayleigh_f64"]) < 1e-4 * scale
+    v = ctx.vec_from(psi0)
+    e64, _ = plan.lanczos(v, tol=1e-12, max_inner=8, max_outer=0, prec=capi.PREC_F64, refine_iters=3)
+    v.close()
+    r64 = plan.lanczos_refinement()
+    assert r64["e_mixed"] == e64 and np.isnan(r64["e_rayleigh_f64"]) and np.isnan(r64["e_refined"])
+
+
 @pytest.mark.parametrize("prec", [capi.PREC_F64, capi.PREC_BF16X3])
 def test_lanczos_graph_replays_equal_eager_launches(ctx, prec, monkeypatch):
     """A small plan's Lanczos runs every H.psi as a replay of its captured

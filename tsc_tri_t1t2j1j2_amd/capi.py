@@ -45,7 +45,7 @@ EXPORTS = [
     "tsc_plan_flops", "tsc_hpsi", "tsc_hpsi_host", "tsc_hpsi_loop", "tsc_hpsi_noalloc", "tsc_hpsi_graph",
     "tsc_plan_last_times",
     "tsc_plan_autotune",
-    "tsc_lanczos", "tsc_lanczos_replays", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_rho_last_times", "tsc_select_states",
+    "tsc_lanczos", "tsc_lanczos_replays", "tsc_lanczos_refinement", "tsc_density_matrix", "tsc_truncation_operator", "tsc_rho_truncate", "tsc_rho_last_times", "tsc_select_states",
     "tsc_update_trun", "tsc_w6j", "tsc_superblock_tasks", "tsc_tasks_info", "tsc_tasks_flops", "tsc_tasks_copy",
     "tsc_model_tasks", "tsc_renorm_tasks", "tsc_tasks_layout", "tsc_tasks_lengths", "tsc_tasks_copy_arena",
     "tsc_tasks_plan", "tsc_tasks_plan_dev", "tsc_tasks_destroy",
@@ -114,6 +114,7 @@ def lib():
         "tsc_plan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, P(c_double)]),
         "tsc_lanczos": (c_int, [c_void_p, c_void_p, c_double, c_int, c_int, c_int, c_int, P(c_double), P(c_int32)]),
         "tsc_lanczos_replays": (c_int, [c_void_p, P(c_int32)]),
+        "tsc_lanczos_refinement": (c_int, [c_void_p, P(c_double)]),
         "tsc_density_matrix": (c_int, [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int, c_void_p]),
         "tsc_truncation_operator": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
                                             c_double, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
@@ -371,6 +372,17 @@ class Plan:
         r = c_int32()
         check(lib().tsc_lanczos_replays(self.h, ctypes.byref(r)))
         return r.value
+
+    def lanczos_refinement(self):
+        """Precision record of the last lanczos(): dict with e_mixed (the
+        working-precision Ritz value), e_rayleigh_f64 and resid_f64 (FP64
+        <psi|H psi> and |H psi - <psi|H psi> psi| of its normalised Ritz
+        vector, from the refinement pass's first product), e_refined (the
+        returned energy) and w_rest (the weight of the refined vector
+        outside the mixed one); all but e_mixed are NaN without refinement."""
+        r = (c_double * 5)()
+        check(lib().tsc_lanczos_refinement(self.h, r))
+        return dict(e_mixed=r[0], e_rayleigh_f64=r[1], resid_f64=r[2], e_refined=r[3], w_rest=r[4])
 
     def lanczos(self, psi, tol=1e-10, max_inner=100, max_outer=4, prec=PREC_F64, refine_iters=0):
         e = c_double()

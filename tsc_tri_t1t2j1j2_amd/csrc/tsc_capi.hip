@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -944,6 +945,11 @@ struct LanczosResult {
   double energy;
   int matvecs;
   int restarts;
+  // alpha_1 = <v_1|H v_1> and beta_1 = |H v_1 - alpha_1 v_1| of the last
+  // outer iteration (v_1: its normalised start vector), and the weight of
+  // the returned Ritz vector outside v_1 (1 - z_1^2, z the normalised
+  // tridiagonal eigenvector)
+  double alpha1 = 0.0, beta1 = 0.0, w_rest = 0.0;
 };
 
 // Plans below this many flops per H.psi replay their products from a
@@ -1057,6 +1063,8 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
       LZ_CALL(dots_re(ctx, vv, 1, V[1], n, &alpha[1]));
     }
     ritzmin = alpha[1];
+    res->alpha1 = alpha[1];
+    res->beta1 = 0.0;
     bool converged_inner = false;
     while (last < max_inner) {
       // w = H v_last - alpha v_last - beta v_{last-1}   (w lives in V[last])
@@ -1071,6 +1079,7 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
         LZ_CALL(dots_re(ctx, vv, 1, V[last], n, &nn));
         beta[last] = std::sqrt(nn);
       }
+      if (last == 1) res->beta1 = beta[1];
       if (std::fabs(beta[last]) <= tol) {
         converged_inner = true;
         done = true;
@@ -1113,6 +1122,11 @@ int lanczos_pass(tsc_plan *p, tsc_vec *psi, double tol, int max_inner, int max_o
     const double e0 = tridiag_lowest(&alpha[1], &beta[1], last);
     tridiag_vector(&alpha[1], &beta[1], last, e0, z.data());
     {
+      double zz = 0.0;
+      for (int i = 0; i < last; ++i) zz += z[i] * z[i];
+      res->w_rest = zz > 0.0 ? std::max(0.0, 1.0 - z[0] * z[0] / zz) : 0.0;
+    }
+    {
       std::vector<const double2 *> vv(V.begin(), V.begin() + last);
       LZ_CALL(combine(ctx, psi->d, 0.0, vv.data(), z.data(), last, n));
       double nn;
@@ -1140,8 +1154,11 @@ extern "C" int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner,
   TSC_REQUIRE(max_outer >= 0 && refine_iters >= 0, "tsc_lanczos iteration counts");
   LanczosResult r{0.0, 0, 0};
   p->lz_replays = 0;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (double &x : p->lz_refine) x = nan;
   int rc = lanczos_pass(p, psi, tol, max_inner, max_outer, prec, prec, &r);
   if (rc) return rc;
+  p->lz_refine[0] = r.energy;
   int mv = r.matvecs, rs = r.restarts;
   if (prec != TSC_PREC_F64 && refine_iters > 0) {
     // Refinement: a Lanczos pass from the Ritz vector psi whose first
@@ -1153,6 +1170,10 @@ extern "C" int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner,
     LanczosResult r2{0.0, 0, 0};
     rc = lanczos_pass(p, psi, tol, std::max(2, refine_iters), 0, TSC_PREC_F64, prec, &r2);
     if (rc) return rc;
+    p->lz_refine[1] = r2.alpha1;
+    p->lz_refine[2] = r2.beta1;
+    p->lz_refine[3] = r2.energy;
+    p->lz_refine[4] = r2.w_rest;
     r.energy = r2.energy;
     mv += r2.matvecs;
   }
@@ -1161,6 +1182,12 @@ extern "C" int tsc_lanczos(tsc_plan *p, tsc_vec *psi, double tol, int max_inner,
     info[0] = mv;
     info[1] = rs;
   }
+  return TSC_OK;
+}
+
+extern "C" int tsc_lanczos_refinement(const tsc_plan *p, double *out) {
+  TSC_REQUIRE(p && out, "tsc_lanczos_refinement arguments");
+  for (int i = 0; i < 5; ++i) out[i] = p->lz_refine[i];
   return TSC_OK;
 }
 

@@ -234,6 +234,11 @@ struct tsc_plan {
   double2 *lz_io[2] = {nullptr, nullptr};
   hipGraphExec_t lz_gexec[3] = {nullptr, nullptr, nullptr};
   int lz_replays = 0;  // H.psi products of the last tsc_lanczos call that ran as graph replays
+  // last tsc_lanczos call: working-precision Ritz value, and of its Ritz
+  // vector the FP64 <psi|H psi> and |H psi - <psi|H psi> psi| from the
+  // refinement pass's first product, the refined energy, and the weight of
+  // the refined vector outside psi (NaN without a refinement pass)
+  double lz_refine[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 };
 
 namespace tsc {

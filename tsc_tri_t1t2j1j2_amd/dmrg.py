@@ -95,6 +95,11 @@ class DMRG:
         # solution); self.shadow holds (step, E_mixed, E_f64)
         self.shadow_f64 = shadow_f64
         self.shadow = []
+        # every mixed solve with a refinement pass: (step, Ritz value of the
+        # mixed pass, FP64 <psi|H psi> and FP64 residual norm of its Ritz
+        # vector, refined energy, weight of the refined vector outside the
+        # mixed one), from the pass's first (FP64) product
+        self.precision_record = []
         # wave_guess: inside a half-sweep, start each Lanczos from the previous
         # ground state carried into the new superblock (wave_trans_LR/RL)
         self.wave_guess = wave_guess
@@ -288,6 +293,45 @@ class DMRG:
         self.shadow.append((self.nstep, E, E64))
         self.timers["shadow"] = self.timers.get("shadow", 0.0) + time.perf_counter() - t0
 
+    def _record_refinement(self, plan):
+        if self._prec_now == capi.PREC_F64 or not self.refine:
+            return
+        r = plan.lanczos_refinement()
+        self.precision_record.append((self.nstep, r["e_mixed"], r["e_rayleigh_f64"], r["resid_f64"],
+                                      r["e_refined"], r["w_rest"]))
+
+    def precision_by_phase(self):
+        """Per phase of the run (warm-up, the half-sweeps, each sweep), from
+        the refinement passes of the mixed solves (no extra work):
+        max_mixed_ritz_dE_per_site, the largest |E_mixed - <psi|H psi>_FP64|
+        / site (the Ritz-energy error of the mixed arithmetic at its own
+        solution, before refinement); max_refined_dE_est_per_site, the
+        largest w_rest * |E_mixed - <psi|H psi>_FP64| / site (the refined
+        energy meets the pass's mixed products only through the refined
+        vector's weight w_rest outside psi: a first-order estimate of its
+        mixed-arithmetic error); max_refine_shift_per_site, the largest
+        (<psi|H psi>_FP64 - E_refined) / site; max_resid_f64, the largest
+        FP64 residual norm of the mixed Ritz vectors; the largest truncation
+        error of the phase and whether the refined estimate stays below it."""
+        trunc = {}
+        for h in self.history:
+            trunc[h[0]] = max(trunc.get(h[0], 0.0), float(h[3]))
+        out = {}
+        for (k, em, ea, res, er, wr) in self.precision_record:
+            tag = self.history[k][0] if k < len(self.history) else "final"
+            o = out.setdefault(tag, {"max_mixed_ritz_dE_per_site": 0.0, "max_refined_dE_est_per_site": 0.0,
+                                     "max_refine_shift_per_site": 0.0, "max_resid_f64": 0.0, "solves": 0})
+            raw = abs(em - ea) / self.N
+            o["max_mixed_ritz_dE_per_site"] = max(o["max_mixed_ritz_dE_per_site"], raw)
+            o["max_refined_dE_est_per_site"] = max(o["max_refined_dE_est_per_site"], float(wr) * raw)
+            o["max_refine_shift_per_site"] = max(o["max_refine_shift_per_site"], (ea - er) / self.N)
+            o["max_resid_f64"] = max(o["max_resid_f64"], float(res))
+            o["solves"] += 1
+        for tag, o in out.items():
+            o["max_trun_err"] = trunc.get(tag, 0.0)
+            o["dE_below_trun_err"] = bool(o["max_refined_dE_est_per_site"] <= o["max_trun_err"])
+        return out
+
     def shadow_by_phase(self):
         """Largest same-superblock |E_mixed - E_FP64| / site per phase of the
         run (warm-up, the half-sweeps, each sweep), from the FP64 shadow
@@ -349,6 +393,7 @@ class DMRG:
                                    refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
         self.timers["lanczos"] += time.perf_counter() - t0
         self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
+        self._record_refinement(plan)
         self._shadow(psi0, plan, tol, E)
         self.hpsi_flops += nmv * plan.flops()
         self.hpsi_calls += nmv
@@ -465,6 +510,7 @@ class DMRG:
                                        refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
             self.timers["lanczos"] += time.perf_counter() - t0
             self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
+            self._record_refinement(plan)
             made["psi0"] = None   # _shadow closes it
             self._shadow(psi0, plan, tol, E)
         finally:

@@ -298,6 +298,11 @@ module tsc_hotpath
       import :: c_int, c_ptr
       type(c_ptr), value :: plan, replays
     end function
+    integer(c_int) function tsc_lanczos_refinement(plan, rec5) bind(c, name="tsc_lanczos_refinement")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double) :: rec5(5)
+    end function
     integer(c_int) function tsc_density_matrix(ctx, psi, nsec, sys_dim, env_dim, side, rho) &
         bind(c, name="tsc_density_matrix")
       import :: c_int, c_ptr, c_int32_t

@@ -112,6 +112,10 @@ def main(argv=None):
         "hpsi_flops_total": d.hpsi_flops, "hpsi_calls": d.hpsi_calls, "steps": len(d.history),
         "guessed_steps": d.guessed, "lanzero": a.lanzero,
     }
+    if d.precision_record:
+        # every mixed solve, from its refinement pass: |E_mixed - <psi|H psi>_FP64| / site and the FP64
+        # residual, per phase, against the phase's truncation error
+        out["mixed_precision_by_phase"] = d.precision_by_phase()
     if d.shadow:
         de = [abs(em - e64) / d.N for (_, em, e64) in d.shadow]
         out["same_path_max_dE_per_site"] = max(de)
