@@ -349,6 +349,13 @@ class RealRun:
                 out["sweep_projection_note"] = (f"{nsw} steps per sweep priced at the measured centre steps "
                                                 "(random-start and guessed); the steps away from the centre are "
                                                 "smaller, so this bounds the sweep from above")
+            # the rho / eigensolve / truncation stage against HBM (its algorithmic bytes over its wall time;
+            # the eigensolves are latency-bound, so this is far below the bandwidth)
+            g = mx[-1]
+            if g.get("truncate_gbs"):
+                out["rho_truncation_roofline"] = {"bound": "hbm", "achieved": g["truncate_gbs"], "peak": HBM_GBS,
+                                                  "unit": "GB/s", "frac": round(g["truncate_gbs"] / HBM_GBS, 6),
+                                                  "bytes": g["truncate_bytes"], "step": g["l"]}
             N = d.N
             de = [abs(mx[0]["E"] - fx[0]["E"]) / N]
             de += [abs(em - e64) / N for (_, em, e64) in windows["mixed"]["shadow"]]

@@ -641,6 +641,7 @@ class DMRG:
                     self.shadow_f64 = l - l0 >= shadow_from
                 t_before = dict(self.timers)
                 r_before = dict(self.rho_parts)
+                b_before = self.trunc_bytes
                 guess = self._guess("LR", prev, self.sys_trun.get(l), self.env_trun.get(N - l - 1))
                 sys, env = self.sys_blocks[l], self.env_blocks[N - l - 2]
                 r = self.step(sys, env, l + 1, l + 2, self.bonds, self.Ne, tol, "sys", kept_max, trun_err,
@@ -656,9 +657,14 @@ class DMRG:
                 parts = {k: round(v - t_before.get(k, 0.0), 3) for k, v in self.timers.items()
                          if v - t_before.get(k, 0.0) > 0.0}
                 truncate_s = {k: round(v - r_before.get(k, 0.0), 4) for k, v in self.rho_parts.items()}
+                # algorithmic HBM bytes of the rho / eigensolve / truncation call (psi read, rho written and
+                # read, eigenvectors written, kept U read) and the rate over its wall time
+                tb = self.trunc_bytes - b_before
+                tsec = self.timers["truncate"] - t_before.get("truncate", 0.0)
                 recs.append({"l": l, "E": r["E"], "seconds": r["seconds"], "matvecs": r["matvecs"],
                              "dim": r["dim"], "nsec": r["nsec"], "kept": r["sys_kept"], "trun_err": r["sys_err"],
-                             "parts_s": parts, "truncate_parts_s": truncate_s})
+                             "parts_s": parts, "truncate_parts_s": truncate_s, "truncate_bytes": tb,
+                             "truncate_gbs": round(tb / tsec / 1e9, 2) if tsec > 0 else None})
                 self.history.append((tag, l, r["E"], r["sys_err"], r["sys_kept"], r["dim"]))
                 self._say(f"{tag} l={l} E={r['E']:.12f} err={r['sys_err']:.3e} kept={r['sys_kept']} "
                           f"dim={r['dim']} nsec={r['nsec']} mv={r['matvecs']} {r['seconds']:.2f}s")
