@@ -117,6 +117,15 @@ def test_mixed_dmrg_six_leg_equals_ed(ctx):
              prec=capi.PREC_BF16X3, refine=2)
     e = d.run()
     assert abs(e - e_ed) / (Lx * Ny) < 1e-10, (e, e_ed)
+    # every solve left its precision record: the mixed Ritz value is off by
+    # the BF16x3 rounding, the refined energy sits at or below the FP64
+    # <psi|H psi> of the mixed vector, and the refinement estimate is tiny
+    assert len(d.precision_record) == len(d.history)
+    for (_, em, ea, res, er, wr) in d.precision_record:
+        assert abs(em - ea) < 1e-3 and res >= 0.0 and 0.0 <= wr <= 1.0
+        assert er <= ea + 1e-12
+    for o in d.precision_by_phase().values():
+        assert o["max_refined_dE_est_per_site"] < 1e-11
 
 
 def test_resident_dmrg_matches_host_operators(ctx):
