@@ -2,8 +2,9 @@
 
 Default workload: the YC6 6x24 t1-t2-J1-J2 cylinder (144 sites, 132
 electrons, t2 = 0.3, J2 = 0.036) at M = 6000 multiplets. A DMRG warm-up
-grows the blocks to the centre (kept_max 1000, its last six steps 6000),
-then the first half-sweep's centre steps run twice from the same blocks:
+grows the blocks to the centre (kept_max 1000, its last seven steps 6000),
+then the first half-sweep's first three centre steps run twice from the
+same blocks:
 mixed precision (BF16x3 H.psi, FP64 Lanczos and refinement) and FP64. At
 the first centre step, before its solve, the H.psi of that real superblock
 is timed: a step is one application of the grouped two-stage contraction
@@ -58,8 +59,8 @@ def parse():
                     help="yc6-m6000: real DMRG superblocks (warm-up, centre window); *-hist: the real centre "
                          "sector histogram (x2 per sector for M=12000) with seeded random block values, H.psi only")
     ap.add_argument("--prec", default="bf16x3", choices=["bf16x3", "f32", "f64"])
-    ap.add_argument("--window-steps", type=int, default=2, help="centre steps timed per precision")
-    ap.add_argument("--ramp", default="1000,6", help="warm-up kept_max and the number of final steps at M "
+    ap.add_argument("--window-steps", type=int, default=3, help="centre steps timed per precision")
+    ap.add_argument("--ramp", default="1000,7", help="warm-up kept_max and the number of final steps at M "
                                                       "(at least the window steps + 4, so that every block the "
                                                       "window reads was grown from an M-block)")
     ap.add_argument("--warmup-lanzero", type=float, default=1e-6)
