@@ -119,6 +119,48 @@ def test_precision_summary_arithmetic():
     assert ph["half"]["max_trun_err"] == 2e-12 and not ph["half"]["dE_below_trun_err"]
 
 
+def test_krylov_budget_fits_free_memory():
+    """The Krylov space of a solve is cut to what free device memory holds
+    (less 4 GiB), with more restarts so that the product budget stays."""
+    from types import SimpleNamespace
+    d = DMRG.__new__(DMRG)
+    d.lanczos_budget, d.budget_cuts = (100, 4), 0
+    n = 1_000_000
+    d.ctx = SimpleNamespace(mem_info=lambda: ((4 << 30) + 20 * 16 * n, 1 << 40))
+    assert d._budget(n) == (19, 26)   # ceil(500 / 19) - 1
+    d.ctx = SimpleNamespace(mem_info=lambda: (1 << 40, 1 << 40))
+    assert d._budget(n) == (100, 4)
+    d.ctx = SimpleNamespace(mem_info=lambda: (1 << 20, 1 << 40))
+    assert d._budget(n) == (8, 62)
+    assert d.budget_cuts == 2
+    d.ctx = SimpleNamespace()   # no memory query: the configured budget
+    assert d._budget(n) == (100, 4)
+
+
+def test_driver_with_a_cut_krylov_space_equals_ed(monkeypatch):
+    """With free memory for only a few Krylov vectors the solves restart
+    and the untruncated run still reaches the exact energy."""
+    ctx = fake_device.install(monkeypatch)
+    Lx, Ny, Ne = 3, 3, 8
+    b = tj_model.bonds(Lx, Ny, 1.0, 0.2, 0.4, 0.016)
+    e_ed = tj_ed.ground_energy(Lx * Ny, Ne, b)
+    cur = {}
+    ctx.mem_info = lambda: ((4 << 30) + 13 * 16 * cur.get("n", 1), 1 << 40)
+    real_budget = DMRG._budget
+
+    def spy(self, n):
+        cur["n"] = n
+        out = real_budget(self, n)
+        assert out[0] == 12
+        return out
+    monkeypatch.setattr(DMRG, "_budget", spy)
+    d = DMRG(ctx, Lx, Ny, Ne=Ne, kept_min=4000, kept_max=4000, sweeps=1, lanzero=1e-10, trun_err=1e-12,
+             native_planner=False)
+    e = d.run()
+    assert d.budget_cuts > 0
+    assert abs(e - e_ed) / (Lx * Ny) < 1e-10, (e, e_ed)
+
+
 def test_renorm_preparation_failure_reaches_the_caller(monkeypatch):
     """The renormalisation bookkeeping built beside the solve: an error
     raised there ends the step with that error, after the solve."""

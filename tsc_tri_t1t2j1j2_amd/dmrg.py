@@ -100,6 +100,7 @@ class DMRG:
         # vector, refined energy, weight of the refined vector outside the
         # mixed one), from the pass's first (FP64) product
         self.precision_record = []
+        self.budget_cuts = 0   # solves whose Krylov space was cut to fit in HBM (_budget)
         # wave_guess: inside a half-sweep, start each Lanczos from the previous
         # ground state carried into the new superblock (wave_trans_LR/RL)
         self.wave_guess = wave_guess
@@ -287,11 +288,33 @@ class DMRG:
         if psi0 is None:
             return
         t0 = time.perf_counter()
-        E64, _ = plan.lanczos(psi0, tol=tol, max_inner=self.lanczos_budget[0], max_outer=self.lanczos_budget[1],
+        inner, outer = self._budget(psi0.n)
+        E64, _ = plan.lanczos(psi0, tol=tol, max_inner=inner, max_outer=outer,
                               prec=capi.PREC_F64)
         psi0.close()
         self.shadow.append((self.nstep, E, E64))
         self.timers["shadow"] = self.timers.get("shadow", 0.0) + time.perf_counter() - t0
+
+    def _budget(self, n):
+        """(max_inner, max_outer) of a solve on n-element vectors: the
+        configured budget, with the Krylov space cut to what free HBM holds
+        (its max_inner + 1 vectors are allocated by the solve, less 4 GiB
+        kept free) and the restarts raised so that the total number of
+        products stays at least the configured one. Large superblocks (M of
+        12000 and more) then run a restarted Lanczos instead of failing to
+        allocate."""
+        inner, outer = self.lanczos_budget
+        mem = getattr(self.ctx, "mem_info", None)
+        if mem is None:
+            return inner, outer
+        free, _ = mem()
+        fit = int((free - (4 << 30)) // (16 * max(int(n), 1))) - 1
+        if fit >= inner:
+            return inner, outer
+        fit = max(fit, 8)
+        total = inner * (outer + 1)
+        self.budget_cuts += 1
+        return fit, max(outer, -(-total // fit) - 1)
 
     def _record_refinement(self, plan):
         if self._prec_now == capi.PREC_F64 or not self.refine:
@@ -388,8 +411,8 @@ class DMRG:
         t0 += self._autotune(plan, psi)   # reported under "autotune", not "lanczos"
         psi0 = self._shadow_start(psi)
         t1 = time.perf_counter()
-        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
-                                   max_outer=self.lanczos_budget[1], prec=self._prec_now,
+        inner, outer = self._budget(psi.n)
+        E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=inner, max_outer=outer, prec=self._prec_now,
                                    refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
         self.timers["lanczos"] += time.perf_counter() - t0
         self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
@@ -505,8 +528,8 @@ class DMRG:
         worker.start()
         try:
             t1 = time.perf_counter()
-            E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=self.lanczos_budget[0],
-                                       max_outer=self.lanczos_budget[1], prec=self._prec_now,
+            inner, outer = self._budget(psi.n)
+            E, (nmv, _) = plan.lanczos(psi, tol=tol, max_inner=inner, max_outer=outer, prec=self._prec_now,
                                        refine_iters=self.refine if self._prec_now != capi.PREC_F64 else 0)
             self.timers["lanczos"] += time.perf_counter() - t0
             self._log_energy(len(sys.sites) + 1, E, nmv, time.perf_counter() - t1)
