@@ -23,3 +23,18 @@ def test_synthetic_superblock_is_hermitian():
     eye = np.eye(n, dtype=complex)
     H = np.stack([apply_numpy(pb, eye[:, i], n) for i in range(n)], axis=1)
     assert np.abs(H - H.conj().T).max() < 1e-10 * np.abs(H).max()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with status 2 and says so,
+    before touching any device."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
